@@ -1,0 +1,246 @@
+"""ctypes binding of libpst_hip.so (include/pst.h).
+
+The product path has no CPU fallback: if the HIP library is missing or the
+device is not a gfx950, every call raises.  Build the library with
+``python -c "import __graft_entry__ as g; g.build()"`` or ``make -C
+py_stoch_amd/csrc``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libpst_hip.so")
+
+METHOD_TAU_LEAPING = 0
+METHOD_FORCED_SSA = 1
+METHOD_PURE_SSA = 2
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+FLAG_OVERFLOW = 1
+FLAG_HALVING = 2
+FLAG_TAU_INF = 4
+FLAG_PTRS = 8
+
+_P_I32 = ctypes.POINTER(ctypes.c_int32)
+_P_F64 = ctypes.POINTER(ctypes.c_double)
+_P_F32 = ctypes.POINTER(ctypes.c_float)
+
+
+class PstNetwork(ctypes.Structure):
+    _fields_ = [
+        ("n_species", ctypes.c_int32),
+        ("n_reactions", ctypes.c_int32),
+        ("a_size", ctypes.c_int32),
+        ("A", _P_I32),
+        ("v_size", ctypes.c_int32),
+        ("V", _P_I32),
+        ("H", _P_I32),
+        ("H_type", _P_I32),
+        ("prop_order", _P_I32),
+        ("prop_step", _P_I32),
+    ]
+
+
+class PstConfig(ctypes.Structure):
+    _fields_ = [
+        ("method", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("n_c", ctypes.c_int32),
+        ("ssa_burst", ctypes.c_int32),
+        ("eta", ctypes.c_double),
+        ("t_max", ctypes.c_double),
+        ("ita", ctypes.c_int32),
+        ("kappa", ctypes.c_int32),
+        ("I", _P_F32),
+        ("E", _P_I32),
+        ("n_sims", ctypes.c_int32),
+        ("sims_per_set", ctypes.c_int32),
+        ("seed", ctypes.c_uint64),
+        ("sim_offset", ctypes.c_uint64),
+    ]
+
+
+class PstStats(ctypes.Structure):
+    _fields_ = [
+        ("events", ctypes.c_int64),
+        ("leaps", ctypes.c_int64),
+        ("rejected_leaps", ctypes.c_int64),
+        ("ssa_steps", ctypes.c_int64),
+        ("flagged", ctypes.c_int64),
+        ("kernel_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        d = {name: int(getattr(self, name)) for name, _ in self._fields_ if name != "kernel_ms"}
+        d["kernel_ms"] = float(self.kernel_ms)
+        return d
+
+
+EXPORTED_SYMBOLS = (
+    "pst_tau_leaping_run",
+    "pst_gillespie_run",
+    "pst_plan_create",
+    "pst_plan_execute",
+    "pst_plan_read_stats",
+    "pst_plan_destroy",
+    "pst_max_network",
+    "pst_last_error",
+    "pst_abi_version",
+)
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    """A libpst_hip call returned an error status."""
+
+
+def load():
+    """Load libpst_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeError(
+            "libpst_hip.so is not built (%s); run __graft_entry__.build() or "
+            "make -C py_stoch_amd/csrc" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    run_args = [ctypes.POINTER(PstNetwork), ctypes.POINTER(PstConfig), _P_F64, _P_I32,
+                ctypes.c_int32, _P_I32, _P_I32, ctypes.POINTER(PstStats)]
+    L.pst_tau_leaping_run.argtypes = run_args
+    L.pst_tau_leaping_run.restype = ctypes.c_int
+    L.pst_gillespie_run.argtypes = run_args
+    L.pst_gillespie_run.restype = ctypes.c_int
+    L.pst_plan_create.argtypes = [ctypes.POINTER(PstNetwork), ctypes.POINTER(PstConfig), _P_F64,
+                                  _P_I32, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+    L.pst_plan_create.restype = ctypes.c_int
+    L.pst_plan_execute.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                   ctypes.c_void_p, ctypes.c_void_p]
+    L.pst_plan_execute.restype = ctypes.c_int
+    L.pst_plan_read_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(PstStats)]
+    L.pst_plan_read_stats.restype = ctypes.c_int
+    L.pst_plan_destroy.argtypes = [ctypes.c_void_p]
+    L.pst_plan_destroy.restype = None
+    L.pst_max_network.argtypes = [_P_I32, _P_I32]
+    L.pst_max_network.restype = None
+    L.pst_last_error.argtypes = []
+    L.pst_last_error.restype = ctypes.c_char_p
+    L.pst_abi_version.argtypes = []
+    L.pst_abi_version.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = load().pst_last_error().decode(errors="replace")
+        raise NativeError("%s failed (status %d): %s" % (what, rc, msg))
+
+
+def _ptr(arr, ctype):
+    return arr.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class _Marshalled:
+    """Keeps the numpy arrays behind a PstNetwork/PstConfig alive."""
+
+    def __init__(self, tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I, E,
+                 n_sims, sims_per_set, seed, sim_offset):
+        N, M = int(tables["N"]), int(tables["M"])
+        self.A = np.ascontiguousarray(tables["A"], np.int32).reshape(-1, 4)
+        self.V = np.ascontiguousarray(tables["V"], np.int32).reshape(-1, 4)
+        self.H = np.ascontiguousarray(tables["H"], np.int32).reshape(N)
+        self.Ht = np.ascontiguousarray(tables["H_type"], np.int32).reshape(N)
+        self.po = np.ascontiguousarray(tables["prop_order"], np.int32).reshape(N, M)
+        self.ps = np.ascontiguousarray(tables["prop_step"], np.int32).reshape(N, M)
+        self.rates = np.ascontiguousarray(np.atleast_2d(np.asarray(rates, np.float64)).reshape(-1, M))
+        self.x0 = np.ascontiguousarray(np.atleast_2d(np.asarray(x0, np.int32)).reshape(-1, N))
+        self.I = np.ascontiguousarray(np.asarray(I, np.float32).reshape(-1))
+        self.E = np.ascontiguousarray(np.asarray(E, np.int32).reshape(-1))
+        self.n_sets = self.rates.shape[0]
+        if self.x0.shape[0] != self.n_sets:
+            raise ValueError("rates has %d rows but x0 has %d" % (self.n_sets, self.x0.shape[0]))
+        if sims_per_set is None:
+            sims_per_set = max(1, int(n_sims)) if self.n_sets == 1 else None
+        if sims_per_set is None:
+            raise ValueError("sims_per_set is required with several parameter rows")
+        self.net = PstNetwork(N, M, self.A.shape[0], _ptr(self.A, ctypes.c_int32), self.V.shape[0],
+                              _ptr(self.V, ctypes.c_int32), _ptr(self.H, ctypes.c_int32),
+                              _ptr(self.Ht, ctypes.c_int32), _ptr(self.po, ctypes.c_int32),
+                              _ptr(self.ps, ctypes.c_int32))
+        self.cfg = PstConfig(int(method), int(precision), int(n_c), int(ssa_burst), float(eta),
+                             float(t_max), self.I.shape[0], self.E.shape[0],
+                             _ptr(self.I, ctypes.c_float), _ptr(self.E, ctypes.c_int32),
+                             int(n_sims), int(sims_per_set), int(seed), int(sim_offset))
+
+
+def simulate(tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64, n_c=10,
+             eta=0.03, ssa_burst=100, t_max=1.0, I=None, E=(0,), n_sims=1, sims_per_set=None,
+             seed=0, sim_offset=0, return_flags=True):
+    """Run simulations on the GPU; returns (O[kappa][ita][n_sims] int32, flags, stats)."""
+    L = load()
+    m = _Marshalled(tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I, E,
+                    n_sims, sims_per_set, seed, sim_offset)
+    O = np.empty((m.E.shape[0], m.I.shape[0], int(n_sims)), np.int32)
+    flags = np.zeros(max(int(n_sims), 1), np.int32) if return_flags else None
+    st = PstStats()
+    fn = L.pst_gillespie_run if method == METHOD_PURE_SSA else L.pst_tau_leaping_run
+    rc = fn(ctypes.byref(m.net), ctypes.byref(m.cfg), _ptr(m.rates, ctypes.c_double),
+            _ptr(m.x0, ctypes.c_int32), m.n_sets, _ptr(O, ctypes.c_int32),
+            _ptr(flags, ctypes.c_int32) if flags is not None else None, ctypes.byref(st))
+    _check(rc, fn.__name__)
+    return O, (flags[: int(n_sims)] if flags is not None else None), st.as_dict()
+
+
+class Plan:
+    """Device-resident run: tables uploaded once, trajectories left in device memory."""
+
+    def __init__(self, tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64,
+                 n_c=10, eta=0.03, ssa_burst=100, t_max=1.0, I=None, E=(0,), n_sims=1,
+                 sims_per_set=None, seed=0, sim_offset=0):
+        L = load()
+        self._m = _Marshalled(tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I,
+                              E, n_sims, sims_per_set, seed, sim_offset)
+        self.n_sims = int(n_sims)
+        self.kappa = self._m.E.shape[0]
+        self.ita = self._m.I.shape[0]
+        h = ctypes.c_void_p()
+        rc = L.pst_plan_create(ctypes.byref(self._m.net), ctypes.byref(self._m.cfg),
+                               _ptr(self._m.rates, ctypes.c_double), _ptr(self._m.x0, ctypes.c_int32),
+                               self._m.n_sets, ctypes.byref(h))
+        _check(rc, "pst_plan_create")
+        self._h = h
+
+    def execute(self, O_dev_ptr, o_stride, flags_dev_ptr=None, stream_ptr=None):
+        """Enqueue one run; O_dev_ptr / flags_dev_ptr are device addresses (int)."""
+        rc = load().pst_plan_execute(self._h, ctypes.c_void_p(int(O_dev_ptr)), int(o_stride),
+                                     ctypes.c_void_p(int(flags_dev_ptr)) if flags_dev_ptr else None,
+                                     ctypes.c_void_p(int(stream_ptr)) if stream_ptr else None)
+        _check(rc, "pst_plan_execute")
+
+    def stats(self):
+        st = PstStats()
+        _check(load().pst_plan_read_stats(self._h, ctypes.byref(st)), "pst_plan_read_stats")
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().pst_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def max_network():
+    n = ctypes.c_int32()
+    m = ctypes.c_int32()
+    load().pst_max_network(ctypes.byref(n), ctypes.byref(m))
+    return int(n.value), int(m.value)

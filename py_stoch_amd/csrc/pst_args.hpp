@@ -1,0 +1,89 @@
+// pst_args.hpp -- kernel argument blocks shared by the launch code and the kernels.
+//
+// A network is passed to the kernels as a dense, zero-padded species x reaction
+// table (NetTab) by value in the kernarg segment: every entry is the same for
+// all lanes of a launch, so the compiler reads it with scalar loads into SGPRs
+// and every branch on it is a scalar branch.  Per-simulation data (state, time,
+// sample cursor, Philox counter) lives in VGPRs for the whole run.
+#pragma once
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+namespace pst {
+
+// Dense network table with capacity NP species x MP reactions.  Rows beyond N
+// and columns beyond M are zero, which makes them inert in every loop.
+template <int NP, int MP>
+struct NetTab {
+  int8_t v[NP * MP];    // net change V = Post - Pre, species-major (d_V, P1_P2.py:8)
+  int8_t ord[NP * MP];  // number of propensity factors of species i in reaction j
+  int8_t stp[NP * MP];  // factor step: 1 -> x(x-1)(x-2)..., 0 -> x^p
+  int8_t H[NP];         // highest order of reaction (d_H, P1_P2.py:9)
+  int8_t Ht[NP];        // d_H_type (P1_P2.py:10)
+  int8_t irs[NP];       // 1 if the species has any Pre entry (I_rs, P1_P2.py:338-346)
+  int8_t E[NP];         // recorded species (d_E, P1_P2.py:13)
+};
+
+struct RunArgs {
+  int32_t N, M;
+  int32_t kappa, ita;
+  int32_t n_c;          // critical threshold (tl_parser.py:43)
+  int32_t ssa_burst;    // SSA steps per fallback burst (P3.py:71)
+  int32_t method;       // 0 tau-leaping, 1 forced SSA, 2 pure SSA
+  int32_t n_sims;
+  int32_t sims_per_set; // simulations sharing one rates/x0 row
+  int32_t precision;    // 0 fp64, 1 mixed
+  double eta;           // tl_parser.py:44
+  double t_max;
+  uint64_t seed;
+  uint64_t sim_offset;
+  int64_t o_stride;     // fastest-axis length of O (>= n_sims)
+  const double* rates;  // [n_sets][M]
+  const int32_t* x0;    // [n_sets][N]
+  const float* I;       // [ita] sample times
+  int32_t* O;           // [kappa][ita][o_stride]
+  int32_t* flags;       // [n_sims] or null
+  unsigned long long* stats;  // 8 counters, see StatIdx
+};
+
+enum StatIdx { ST_EVENTS = 0, ST_LEAPS, ST_REJECTED, ST_SSA, ST_FLAGGED, ST_COUNT = 8 };
+
+// host-side copy of a network in dense form, built by the C-ABI layer
+struct HostNet {
+  int N, M, kappa;
+  int8_t v[64 * 256];
+  int8_t ord[64 * 256];
+  int8_t stp[64 * 256];
+  int8_t H[64], Ht[64], irs[64];
+  int8_t E[64];
+};
+
+// size classes compiled into the library
+constexpr int NUM_CLASSES = 3;
+constexpr int CLASS_NP[NUM_CLASSES] = {4, 8, 16};
+constexpr int CLASS_MP[NUM_CLASSES] = {4, 8, 16};
+
+template <int NP, int MP>
+inline NetTab<NP, MP> make_tab(const HostNet& h) {
+  NetTab<NP, MP> t;
+  for (int i = 0; i < NP * MP; ++i) { t.v[i] = 0; t.ord[i] = 0; t.stp[i] = 0; }
+  for (int i = 0; i < NP; ++i) { t.H[i] = 0; t.Ht[i] = 0; t.irs[i] = 0; t.E[i] = 0; }
+  for (int i = 0; i < h.N; ++i) {
+    for (int j = 0; j < h.M; ++j) {
+      t.v[i * MP + j] = h.v[i * h.M + j];
+      t.ord[i * MP + j] = h.ord[i * h.M + j];
+      t.stp[i * MP + j] = h.stp[i * h.M + j];
+    }
+    t.H[i] = h.H[i];
+    t.Ht[i] = h.Ht[i];
+    t.irs[i] = h.irs[i];
+  }
+  for (int q = 0; q < h.kappa; ++q) t.E[q] = h.E[q];
+  return t;
+}
+
+// launchers, one per translation unit (precision)
+int launch_fp64(const HostNet& net, const RunArgs& ra, hipStream_t stream);
+int launch_mixed(const HostNet& net, const RunArgs& ra, hipStream_t stream);
+
+}  // namespace pst

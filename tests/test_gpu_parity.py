@@ -1,0 +1,75 @@
+"""FP64 GPU path vs the CPU oracle: bit-exact int32 trajectories, flags and
+counters on the same Philox stream (every test marked gpu)."""
+import numpy as np
+import pytest
+
+from py_stoch_amd import models
+from py_stoch_amd.mod.parser.tl_parser import tables_of
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(native, oracle_mod, spn, E, t_max, ita, U, method, seed=7, precision=0, **extra):
+    a = models.make_args(spn, E=E, t_max=t_max, ita=ita, U=U)
+    tabs = tables_of(a)
+    rates = a.rate_fn(a.c)
+    kw = dict(method=method, t_max=t_max, I=a.I, E=a.E, n_sims=U, seed=seed, **extra)
+    print("case: U=%d ita=%d method=%d seed=%d" % (U, ita, method, seed), flush=True)
+    O_g, f_g, s_g = native.simulate(tabs, rates, a.x_0, precision=precision, **kw)
+    O_c, f_c, s_c = oracle_mod.simulate(tabs, rates, a.x_0, **kw)
+    assert O_g.shape == O_c.shape
+    mism = np.argwhere(O_g != O_c)
+    assert mism.shape[0] == 0, "first mismatch %s gpu=%d cpu=%d" % (
+        mism[0].tolist(), O_g[tuple(mism[0])], O_c[tuple(mism[0])])
+    assert np.array_equal(f_g, f_c)
+    for k in ("events", "leaps", "rejected_leaps", "ssa_steps", "flagged"):
+        assert s_g[k] == s_c[k], (k, s_g[k], s_c[k])
+    return O_g, s_g
+
+
+def test_birth_death_pure_ssa(native, oracle_mod):
+    _compare(native, oracle_mod, models.birth_death(), [0], 100.0, 101, 1024, native.METHOD_PURE_SSA, seed=0)
+
+
+def test_birth_death_forced_ssa(native, oracle_mod):
+    _compare(native, oracle_mod, models.birth_death(), [0], 100.0, 101, 1024, native.METHOD_FORCED_SSA, seed=0)
+
+
+def test_birth_death_tau_leaping(native, oracle_mod):
+    _compare(native, oracle_mod, models.birth_death(), [0], 100.0, 101, 1000, native.METHOD_TAU_LEAPING)
+
+
+def test_lotka_volterra_tau_leaping(native, oracle_mod):
+    _, st = _compare(native, oracle_mod, models.lotka_volterra(), [0, 1], 10.0, 1000, 2048,
+                     native.METHOD_TAU_LEAPING, seed=2)
+    assert st["leaps"] > 0
+
+
+def test_schlogl_tau_leaping(native, oracle_mod):
+    _, st = _compare(native, oracle_mod, models.schlogl(), [0], 10.0, 100, 1024, native.METHOD_TAU_LEAPING, seed=3)
+    assert st["leaps"] > 0 and st["ssa_steps"] > 0
+
+
+def test_schlogl_pure_ssa(native, oracle_mod):
+    _compare(native, oracle_mod, models.schlogl(), [0], 2.0, 21, 512, native.METHOD_PURE_SSA, seed=3)
+
+
+def test_ragged_and_tiny_sim_counts(native, oracle_mod):
+    for U in (1, 2, 3, 63, 65, 257):
+        _compare(native, oracle_mod, models.lotka_volterra(), [1, 0], 2.0, 17, U, native.METHOD_TAU_LEAPING, seed=U)
+
+
+def test_absorbing_state(native, oracle_mod):
+    # pure death from 30: every simulation reaches the absorbing a0 == 0 state
+    spn = models.birth_death(birth=0.0, death=1.0, x0=30)
+    O, _ = _compare(native, oracle_mod, spn, [0], 50.0, 51, 300, native.METHOD_TAU_LEAPING, seed=5)
+    assert (O[0, -1] == 0).all()
+
+
+def test_sim_offset_reproducible(native, oracle_mod):
+    a = models.make_args(models.lotka_volterra(), E=[0, 1], t_max=1.0, ita=11, U=128)
+    tabs, rates = tables_of(a), a.rate_fn(a.c)
+    kw = dict(method=0, t_max=1.0, I=a.I, E=a.E, seed=9)
+    full, _, _ = native.simulate(tabs, rates, a.x_0, n_sims=128, **kw)
+    tail, _, _ = native.simulate(tabs, rates, a.x_0, n_sims=64, sim_offset=64, **kw)
+    assert np.array_equal(full[:, :, 64:], tail)
