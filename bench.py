@@ -1,0 +1,232 @@
+"""Benchmark: reaction events/s of the simulator hot path on one MI355X.
+
+Workload (one "step" = one full run of the workload, trajectories written to
+device memory): Lotka-Volterra tau-leaping with SSA fallback,
+X -> 2X (10), X + Y -> 2Y (0.01), Y -> 0 (10), X0 = Y0 = 1000,
+epsilon = 0.03, n_c = 10, SSA burst 100, 65536 simulations, t_end = 10,
+1000 sample times, both species recorded.
+
+``value`` is the mixed-precision path (FP32 propensities / tau bound, FP64
+clock and reaction choice).  The same workload is also timed on the plain
+FP64 path (``fp64_plain``), the mixed result is checked against it (relative
+error of per-species means and variances and the KS distance at every sample
+time), and the CPU oracle (the FP64 method in C, OpenMP over simulations)
+is timed on a bounded sample of the same workload (``cpu_baseline``).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "reaction events/s, 1 GPU: mixed-prec batched vs FP64 plain at ≤1% moment err"
+VALU_PEAK = 256 * 4 * 16 * 2.4e9      # lane-instructions/s: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz
+HBM_PEAK = 8.0e12                      # bytes/s
+
+U = 65536
+T_END = 10.0
+ITA = 1000
+SEED = 2
+
+
+def work_model(tabs, kappa, st):
+    """Algorithmic VALU lane-operations of one run from the kernel's counters.
+
+    Weights per counted unit (add/mul/compare/convert = 1, divide/log/exp = 4,
+    Philox4x32-10 block = 140):
+      uniform              74   half a Philox block + conversion
+      PTRS iteration       62   beyond its two uniforms
+      inversion step        3
+      leap decision        P + 2M + 23 nnz(V) + 5   propensities, a0, critical
+                                                   test, mu/sigma^2, Cao bound
+      accepted leap        2 nnz(V) + N + 1         state update and checks
+      rejected leap        2 nnz(V) + N
+      SSA step             P + 4M + 13 + nnz(V)/M   propensities, a0, tau,
+                                                   selection, update
+      sample row           8 kappa
+    with P = sum_j (1 + 2 order_j).
+    """
+    po = np.asarray(tabs["prop_order"])
+    N, M = int(tabs["N"]), int(tabs["M"])
+    nnz = int(np.asarray(tabs["V"]).reshape(-1, 4).shape[0])
+    P = int(sum(1 + 2 * int(po[:, j].sum()) for j in range(M)))
+    w = (74 * st["uniforms"] + 62 * st["ptrs_iterations"] + 3 * st["inversion_steps"]
+         + (P + 2 * M + 23 * nnz + 5) * st["leap_decisions"]
+         + (2 * nnz + N + 1) * st["leaps"] + (2 * nnz + N) * st["rejected_leaps"]
+         + (P + 4 * M + 13 + nnz / M) * st["ssa_steps"] + 8 * kappa * st["sample_rows"])
+    return float(w)
+
+
+def accuracy(O_mixed, O_ref):
+    """Max over species and sample times of the relative error of the mean and
+    variance, and of the two-sample KS distance (torch tensors [kappa][ita][U])."""
+    import torch
+    a = O_mixed.double()
+    b = O_ref.double()
+    ma, mb = a.mean(-1), b.mean(-1)
+    va, vb = a.var(-1), b.var(-1)
+    mask_m = mb.abs() > 0
+    mask_v = vb > 0
+    mean_err = ((ma - mb).abs() / mb.abs().clamp_min(1e-300))[mask_m].max().item() if mask_m.any() else 0.0
+    var_err = ((va - vb).abs() / vb.clamp_min(1e-300))[mask_v].max().item() if mask_v.any() else 0.0
+    k, f, n = a.shape
+    sa = torch.sort(O_mixed.reshape(k * f, n), dim=-1).values.contiguous()
+    sb = torch.sort(O_ref.reshape(k * f, n), dim=-1).values.contiguous()
+    vals = torch.cat([sa, sb], dim=-1).contiguous()
+    fa = torch.searchsorted(sa, vals, right=True).double() / n
+    fb = torch.searchsorted(sb, vals, right=True).double() / n
+    ks = (fa - fb).abs().max().item()
+    # the same statistics at the last sample time
+    last = {"mean_rel_err": float(((ma[:, -1] - mb[:, -1]).abs() / mb[:, -1].abs()).max().item()),
+            "var_rel_err": float(((va[:, -1] - vb[:, -1]).abs() / vb[:, -1]).max().item())}
+    return {"mean_rel_err_max": float(mean_err), "var_rel_err_max": float(var_err), "ks_max": float(ks),
+            "at_t_end": last}
+
+
+def time_plan(plan, O, K, W, stream):
+    import torch
+    for _ in range(W):
+        plan.execute(O.data_ptr(), U, None, stream.cuda_stream)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(K):
+        plan.execute(O.data_ptr(), U, None, stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = plan.stats()
+    return (t1 - t0), e0.elapsed_time(e1) / K, st
+
+
+def cpu_baseline(tabs, rates, x0, I, E, budget_s=15.0):
+    from oracle import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    probe = 256
+    t0 = time.perf_counter()
+    oracle.simulate(tabs, rates, x0, method=0, t_max=T_END, I=I, E=E, n_sims=probe, seed=SEED,
+                    threads=threads)
+    dt = time.perf_counter() - t0
+    n = int(min(U, max(probe, (budget_s / max(dt, 1e-6)) * probe)))
+    n = max(threads, (n // threads) * threads)
+    t0 = time.perf_counter()
+    _, _, st = oracle.simulate(tabs, rates, x0, method=0, t_max=T_END, I=I, E=E, n_sims=n, seed=SEED,
+                               threads=threads)
+    dt = time.perf_counter() - t0
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": st["events"] / dt, "unit": "reaction events/s", "cores": threads, "kind": "port",
+            "sample": "first %d of %d simulations of the same workload, FP64 oracle (C, OpenMP), "
+                      "%.2f s, %s" % (n, U, dt, cpu_model)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: simulations are independent, so N GPUs would each run "
+              "their own range of simulation ids; multi-GPU runs are not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from py_stoch_amd import _native, models
+    from py_stoch_amd.mod.parser.tl_parser import tables_of
+
+    a = models.make_args(models.lotka_volterra(), E=[0, 1], t_max=T_END, ita=ITA, U=U)
+    tabs = tables_of(a)
+    rates = a.rate_fn(a.c)
+    kappa = 2
+    common = dict(method=_native.METHOD_TAU_LEAPING, n_c=10, eta=0.03, ssa_burst=100, t_max=T_END,
+                  I=a.I, E=a.E, n_sims=U, seed=SEED)
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    print("bench: building plans", file=sys.stderr, flush=True)
+    p_mixed = _native.Plan(tabs, rates, a.x_0, precision=_native.PRECISION_MIXED, **common)
+    p_fp64 = _native.Plan(tabs, rates, a.x_0, precision=_native.PRECISION_FP64, **common)
+    O_m = torch.empty((kappa, ITA, U), dtype=torch.int32, device=dev)
+    O_f = torch.empty((kappa, ITA, U), dtype=torch.int32, device=dev)
+
+    print("bench: mixed precision, %d warmup + %d timed steps" % (args.warmup, args.steps),
+          file=sys.stderr, flush=True)
+    wall_m, kern_ms_m, st_m = time_plan(p_mixed, O_m, args.steps, args.warmup, stream)
+    k_f = max(1, args.steps // 2)
+    print("bench: plain FP64, 1 warmup + %d timed steps" % k_f, file=sys.stderr, flush=True)
+    wall_f, kern_ms_f, st_f = time_plan(p_fp64, O_f, k_f, 1, stream)
+
+    ev = st_m["events"]
+    value = ev * args.steps / wall_m
+    fp64_value = st_f["events"] * k_f / wall_f
+    acc = accuracy(O_m, O_f)
+    acc["pass"] = bool(acc["mean_rel_err_max"] <= 0.01 and acc["var_rel_err_max"] <= 0.01 and acc["ks_max"] <= 0.02)
+
+    W = work_model(tabs, kappa, st_m)
+    t_k = kern_ms_m / 1e3
+    traj_bytes = 4.0 * kappa * ITA * U
+    roof = {"bound": "valu", "achieved": W / t_k / 1e12, "peak": VALU_PEAK / 1e12, "unit": "Tlane-op/s",
+            "frac": W / t_k / VALU_PEAK, "traffic": None,
+            "work_lane_ops": W, "kernel_ms": kern_ms_m,
+            "hbm": {"algorithmic_bytes": traj_bytes, "achieved_GBps": traj_bytes / t_k / 1e9,
+                    "frac": traj_bytes / t_k / HBM_PEAK}}
+    prof_traffic = os.path.join(ROOT, "profiles", "traffic_lv65536_mixed.json")
+    if os.path.exists(prof_traffic):
+        with open(prof_traffic) as f:
+            roof["traffic"] = json.load(f).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        print("bench: CPU oracle baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(tabs, rates, a.x_0, a.I, a.E)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "reaction events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall_m / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "lotka-volterra tau-leaping, 65536 simulations, t_end=10, 1000 samples, "
+                               "eps=0.03, n_c=10, SSA burst 100, mixed precision",
+                   "simulations": U, "t_end": T_END, "samples": ITA, "species_recorded": kappa,
+                   "seed": SEED, "events_per_step": ev},
+        "fp64_plain": {"value": fp64_value, "ms_per_step": wall_f / k_f * 1e3, "kernel_ms": kern_ms_f,
+                       "events_per_step": st_f["events"]},
+        "speedup_vs_fp64": value / fp64_value,
+        "accuracy_vs_fp64": acc,
+        "counters": {k: v for k, v in st_m.items() if k != "kernel_ms"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

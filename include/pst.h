@@ -98,6 +98,11 @@ typedef struct pst_stats {
   int64_t rejected_leaps;     /* tau halvings                                 */
   int64_t ssa_steps;          /* applied SSA reactions                        */
   int64_t flagged;            /* simulations that ended with a flag           */
+  int64_t uniforms;           /* uniform variates drawn                       */
+  int64_t ptrs_iterations;    /* Poisson PTRS rejection-loop iterations       */
+  int64_t inversion_steps;    /* Poisson inversion search steps               */
+  int64_t leap_decisions;     /* tau-leap decisions taken (P1_P2 bodies)      */
+  int64_t sample_rows;        /* sample rows written (per recorded row of O)  */
   double kernel_ms;           /* device time of the simulation kernel         */
 } pst_stats;
 

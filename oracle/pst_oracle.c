@@ -66,19 +66,25 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 }
 
 /*
- * The per-simulation stream.  Block b of simulation s is
- * Philox(ctr = (b_lo, b_hi, s_lo, s_hi), key = (seed_lo, seed_hi)).
- * An FP64 uniform uses two consecutive words of a block (two per block):
+ * The per-simulation stream.  Every random draw comes from a Philox block
+ *   Philox(ctr = (c_lo, c_hi | it << 16, s_lo, s_hi), key = (seed_lo, seed_hi))
+ * of simulation s, where c is a per-simulation block counter and it a retry
+ * index (0 except for PTRS rejection iterations).  Blocks are taken in groups
+ * at fixed points of the algorithm, each draw at a fixed word position:
+ *   SSA step:            1 block  (c = b++): rho1 = U(w0,w1), rho2 = U(w2,w3)
+ *   critical tau2/pick:  1 block  (c = b++): rho1 = U(w0,w1), rho  = U(w2,w3)
+ *   Poisson attempt:     M blocks (c = b .. b+M-1, b += M): reaction j draws
+ *                        from block b+j (inversion u = U(w0,w1); PTRS
+ *                        iteration it: U = U(w0,w1), V = U(w2,w3) of block
+ *                        (b+j, it))
+ * An FP64 uniform from two words:
  *   m = (w_a >> 5) * 2^26 + (w_b >> 6),   u = (m + 1) * 2^-53  in (0, 1]
  * which keeps curand_uniform's (0, 1] convention (P1_P2.py:201, P3.py:114).
  */
 typedef struct {
     uint32_t key[2];
     uint64_t sim;
-    uint64_t draws;        /* FP64 uniforms consumed so far */
-    uint64_t cached_block;
-    uint32_t words[4];
-    int have_cache;
+    uint64_t b;            /* next block counter */
 } orc_rng;
 
 static void rng_init(orc_rng* g, uint64_t seed, uint64_t sim)
@@ -86,34 +92,32 @@ static void rng_init(orc_rng* g, uint64_t seed, uint64_t sim)
     g->key[0] = (uint32_t)seed;
     g->key[1] = (uint32_t)(seed >> 32);
     g->sim = sim;
-    g->draws = 0;
-    g->have_cache = 0;
-    g->cached_block = 0;
+    g->b = 0;
 }
 
-static double rng_uniform(orc_rng* g)
+static void rng_block(const orc_rng* g, uint64_t c, uint32_t it, uint32_t w[4])
 {
-    uint64_t blk = g->draws >> 1;
-    int half = (int)(g->draws & 1u);
-    if (!g->have_cache || g->cached_block != blk) {
-        uint32_t ctr[4] = {(uint32_t)blk, (uint32_t)(blk >> 32),
-                           (uint32_t)g->sim, (uint32_t)(g->sim >> 32)};
-        orc_philox4x32_10(ctr, g->key, g->words);
-        g->cached_block = blk;
-        g->have_cache = 1;
-    }
-    g->draws++;
-    uint32_t wa = g->words[2 * half], wb = g->words[2 * half + 1];
+    uint32_t ctr[4] = {(uint32_t)c, (uint32_t)(c >> 32) | (it << 16),
+                       (uint32_t)g->sim, (uint32_t)(g->sim >> 32)};
+    orc_philox4x32_10(ctr, g->key, w);
+}
+
+static double u53(uint32_t wa, uint32_t wb)
+{
     uint64_t m = ((uint64_t)(wa >> 5) << 26) | (uint64_t)(wb >> 6);
     return (double)(m + 1u) * 0x1p-53;
 }
 
-/* exported for the known-answer tests */
+/* exported for the known-answer tests: 2 uniforms per block, blocks 0, 1, ... */
 void orc_uniform_stream(uint64_t seed, uint64_t sim, int64_t n, double* out)
 {
     orc_rng g;
     rng_init(&g, seed, sim);
-    for (int64_t i = 0; i < n; ++i) out[i] = rng_uniform(&g);
+    uint32_t w[4];
+    for (int64_t i = 0; i < n; ++i) {
+        if ((i & 1) == 0) rng_block(&g, (uint64_t)(i >> 1), 0, w);
+        out[i] = (i & 1) ? u53(w[2], w[3]) : u53(w[0], w[1]);
+    }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -209,17 +213,20 @@ double orc_lgamma_int(double x)   /* x = k + 1, k >= 0 integral */
 }
 
 /*
- * Poisson(lambda) sampler replacing curand_poisson (P1_P2.py:239).
+ * Poisson(lambda) sampler replacing curand_poisson (P1_P2.py:239), drawing
+ * from block c of the stream (retry index it for PTRS iteration it).
  *   lambda <= 0      -> 0
  *   lambda < 10      -> sequential-search inversion (Devroye 1986, X.3)
  *   lambda >= 10     -> PTRS transformed rejection (Hoermann 1993)
  * Returns -1 when the PTRS loop limit is reached.
  */
-static int64_t orc_poisson(orc_rng* g, double lambda)
+static int64_t orc_poisson(const orc_rng* g, uint64_t c, double lambda)
 {
+    uint32_t w[4];
     if (!(lambda > 0.0)) return 0;
     if (lambda < 10.0) {
-        double u = rng_uniform(g);
+        rng_block(g, c, 0, w);
+        double u = u53(w[0], w[1]);
         double p = orc_exp(-lambda);
         double F = p;
         int64_t k = 0;
@@ -234,29 +241,29 @@ static int64_t orc_poisson(orc_rng* g, double lambda)
     double slam = orc_exp(0.5 * loglam);
     double b = 0.931 + 2.53 * slam;
     double a = -0.059 + 0.02483 * b;
-    double invalpha = 1.1239 + 1.1328 / (b - 3.4);
     double vr = 0.9277 - 3.6224 / (b - 2.0);
-    double log_invalpha = orc_log(invalpha);
     for (int it = 0; it < ORC_PTRS_MAX_ITERS; ++it) {
-        double U = rng_uniform(g) - 0.5;
-        double V = rng_uniform(g);
+        rng_block(g, c, (uint32_t)it, w);
+        double U = u53(w[0], w[1]) - 0.5;
+        double V = u53(w[2], w[3]);
         double us = 0.5 - fabs(U);
         double kf = floor((2.0 * a / us + b) * U + lambda + 0.43);
         if (us >= 0.07 && V <= vr) return (int64_t)kf;
         if (kf < 0.0 || (us < 0.013 && V > us)) continue;
-        double lhs = orc_log(V) + log_invalpha - orc_log(a / (us * us) + b);
+        double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+        double lhs = orc_log(V) + orc_log(invalpha) - orc_log(a / (us * us) + b);
         double rhs = -lambda + kf * loglam - orc_lgamma_int(kf + 1.0);
         if (lhs <= rhs) return (int64_t)kf;
     }
     return -1;
 }
 
-/* exported for the distribution tests */
+/* exported for the distribution tests: sample i uses block counter i */
 void orc_poisson_stream(uint64_t seed, uint64_t sim, double lambda, int64_t n, int64_t* out)
 {
     orc_rng g;
     rng_init(&g, seed, sim);
-    for (int64_t i = 0; i < n; ++i) out[i] = orc_poisson(&g, lambda);
+    for (int64_t i = 0; i < n; ++i) out[i] = orc_poisson(&g, (uint64_t)i, lambda);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -383,9 +390,12 @@ static int simulate_one(const orc_network* net, const double* k, const int32_t* 
             double a0 = 0.0;
             for (int j = 0; j < M; ++j) a0 = a0 + a[j];
             if (a0 == 0.0) { fill_rest(c, O, U, sim, F, x, 0); return flags; }
-            double rho1 = rng_uniform(&g);
+            uint32_t w[4];
+            rng_block(&g, g.b, 0, w);
+            g.b += 1;
+            double rho1 = u53(w[0], w[1]);
             double tau = (1.0 / a0) * orc_log(1.0 / rho1);
-            double rho = rng_uniform(&g);
+            double rho = u53(w[2], w[3]);
             double rc = 0.0;
             int j = -1, last = -1;
             for (int r = 0; r < M; ++r) {
@@ -467,8 +477,11 @@ static int simulate_one(const orc_network* net, const double* k, const int32_t* 
                 double a0c = 0.0;
                 for (int j = 0; j < M; ++j) if (Xeta[j] == 1) a0c = a0c + a[j];
                 double tau2 = INFINITY;
+                uint32_t wc[4] = {0, 0, 0, 0};
                 if (a0c > 0.0) {
-                    double rho1 = rng_uniform(&g);
+                    rng_block(&g, g.b, 0, wc);
+                    g.b += 1;
+                    double rho1 = u53(wc[0], wc[1]);
                     tau2 = (1.0 / a0c) * orc_log(1.0 / rho1);
                 }
                 double tau = (double)fminf((float)tau1, (float)tau2);
@@ -476,7 +489,7 @@ static int simulate_one(const orc_network* net, const double* k, const int32_t* 
                     /* SingleCriticalReaction (P1_P2.py:488-507); the reference
                      * falls off the end if round-off leaves rcount < rho: the
                      * last critical reaction with a_j > 0 is taken instead. */
-                    double rho = rng_uniform(&g);
+                    double rho = u53(wc[2], wc[3]);
                     double rc = 0.0;
                     int jc = -1, last = -1;
                     for (int r = 0; r < M; ++r) {
@@ -498,11 +511,13 @@ static int simulate_one(const orc_network* net, const double* k, const int32_t* 
                 int valid = 0;
                 for (int h = 0; h < ORC_MAX_HALVINGS; ++h) {
                     int bad = 0;
+                    uint64_t grp = g.b;
+                    g.b += (uint64_t)M;
                     for (int j = 0; j < M; ++j) {
                         if (Xeta[j] == 0) {
                             double lam = a[j] * tau;
                             if (lam >= ORC_LAMBDA_LIMIT) { bad = ORC_FLAG_OVERFLOW; break; }
-                            int64_t kk = orc_poisson(&g, lam);
+                            int64_t kk = orc_poisson(&g, grp + (uint64_t)j, lam);
                             if (kk < 0) { bad = ORC_FLAG_PTRS; break; }
                             K[j] = (int32_t)kk;
                         }
@@ -556,10 +571,13 @@ static int simulate_one(const orc_network* net, const double* k, const int32_t* 
                 double a0 = 0.0;
                 for (int j = 0; j < M; ++j) a0 = a0 + a[j];
                 if (a0 == 0.0) { fill_rest(c, O, U, sim, F, x, 0); return flags; }
-                double rho1 = rng_uniform(&g);
+                uint32_t w[4];
+                rng_block(&g, g.b, 0, w);
+                g.b += 1;
+                double rho1 = u53(w[0], w[1]);
                 double tau = (1.0 / a0) * orc_log(1.0 / rho1);
                 /* SingleCriticalReaction, P3.py:179-194 (divides each term by a0) */
-                double rho = rng_uniform(&g);
+                double rho = u53(w[2], w[3]);
                 double rc = 0.0;
                 int j = -1, last = -1;
                 for (int r = 0; r < M; ++r) {

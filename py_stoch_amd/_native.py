@@ -71,6 +71,11 @@ class PstStats(ctypes.Structure):
         ("rejected_leaps", ctypes.c_int64),
         ("ssa_steps", ctypes.c_int64),
         ("flagged", ctypes.c_int64),
+        ("uniforms", ctypes.c_int64),
+        ("ptrs_iterations", ctypes.c_int64),
+        ("inversion_steps", ctypes.c_int64),
+        ("leap_decisions", ctypes.c_int64),
+        ("sample_rows", ctypes.c_int64),
         ("kernel_ms", ctypes.c_double),
     ]
 

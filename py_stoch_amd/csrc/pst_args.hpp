@@ -33,6 +33,8 @@ struct RunArgs {
   int32_t n_sims;
   int32_t sims_per_set; // simulations sharing one rates/x0 row
   int32_t precision;    // 0 fp64, 1 mixed
+  int32_t ssa_slice;    // SSA steps per pass while other lanes wait to decide (launch tuning)
+  int32_t pad1;
   double eta;           // tl_parser.py:44
   double t_max;
   uint64_t seed;
@@ -43,10 +45,14 @@ struct RunArgs {
   const float* I;       // [ita] sample times
   int32_t* O;           // [kappa][ita][o_stride]
   int32_t* flags;       // [n_sims] or null
-  unsigned long long* stats;  // 8 counters, see StatIdx
+  unsigned long long* stats;  // ST_COUNT counters, see StatIdx
 };
 
-enum StatIdx { ST_EVENTS = 0, ST_LEAPS, ST_REJECTED, ST_SSA, ST_FLAGGED, ST_COUNT = 8 };
+enum StatIdx {
+  ST_EVENTS = 0, ST_LEAPS, ST_REJECTED, ST_SSA, ST_FLAGGED,
+  ST_UNIFORMS, ST_PTRS, ST_INV, ST_DECISIONS, ST_SAMPLES,
+  ST_COUNT = 16
+};
 
 // host-side copy of a network in dense form, built by the C-ABI layer
 struct HostNet {
@@ -81,6 +87,60 @@ inline NetTab<NP, MP> make_tab(const HostNet& h) {
   for (int q = 0; q < h.kappa; ++q) t.E[q] = h.E[q];
   return t;
 }
+
+// Kernel-side view of a NetTab: the table bytes packed four to a 32-bit word
+// and made opaque to the optimizer at the top of each simulation-loop
+// iteration (refresh()), so that the per-use byte extractions stay inside the
+// loop as cheap scalar bit-field extracts instead of being hoisted into one
+// SGPR per table entry (which spills).
+template <int NP, int MP>
+struct TabView {
+  static constexpr int NW = (NP * MP + 3) / 4;
+  static constexpr int NS = (NP + 3) / 4;
+  uint32_t v[NW], ord[NW], stp[NW];
+  uint32_t H[NS], Ht[NS], irs[NS], E[NS];
+
+  __device__ __forceinline__ void load(const NetTab<NP, MP>& t) {
+    __builtin_memcpy(v, t.v, sizeof(t.v));
+    __builtin_memcpy(ord, t.ord, sizeof(t.ord));
+    __builtin_memcpy(stp, t.stp, sizeof(t.stp));
+    __builtin_memcpy(H, t.H, sizeof(t.H));
+    __builtin_memcpy(Ht, t.Ht, sizeof(t.Ht));
+    __builtin_memcpy(irs, t.irs, sizeof(t.irs));
+    __builtin_memcpy(E, t.E, sizeof(t.E));
+  }
+  // An empty asm with a VGPR operand hides the value from the optimizer;
+  // readfirstlane brings it back to an SGPR (all lanes hold the same word).
+  static __device__ __forceinline__ uint32_t opaque(uint32_t w) {
+    asm volatile("" : "+v"(w));
+    return __builtin_amdgcn_readfirstlane(w);
+  }
+  __device__ __forceinline__ void refresh() {
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      v[w] = opaque(v[w]);
+      ord[w] = opaque(ord[w]);
+      stp[w] = opaque(stp[w]);
+    }
+#pragma unroll
+    for (int w = 0; w < NS; ++w) {
+      H[w] = opaque(H[w]);
+      Ht[w] = opaque(Ht[w]);
+      irs[w] = opaque(irs[w]);
+      E[w] = opaque(E[w]);
+    }
+  }
+  static __device__ __forceinline__ int sbyte(const uint32_t* w, int idx) {
+    return (int)(int8_t)(w[idx >> 2] >> (8 * (idx & 3)));
+  }
+  __device__ __forceinline__ int V(int i, int j) const { return sbyte(v, i * MP + j); }
+  __device__ __forceinline__ int Ord(int i, int j) const { return sbyte(ord, i * MP + j); }
+  __device__ __forceinline__ int Stp(int i, int j) const { return sbyte(stp, i * MP + j); }
+  __device__ __forceinline__ int Hv(int i) const { return sbyte(H, i); }
+  __device__ __forceinline__ int Htv(int i) const { return sbyte(Ht, i); }
+  __device__ __forceinline__ int Irs(int i) const { return sbyte(irs, i); }
+  __device__ __forceinline__ int Ev(int q) const { return sbyte(E, q); }
+};
 
 // launchers, one per translation unit (precision)
 int launch_fp64(const HostNet& net, const RunArgs& ra, hipStream_t stream);

@@ -218,6 +218,7 @@ int pst_plan_create(const pst_network* net, const pst_config* cfg, const double*
   ra.I = p->d_I;
   ra.stats = p->d_stats;
   ra.precision = cfg->precision;
+  ra.ssa_slice = 4;
   *out = p;
   return PST_OK;
 }
@@ -262,6 +263,11 @@ int pst_plan_read_stats(pst_plan* p, pst_stats* st) {
   st->rejected_leaps = (int64_t)h[pst::ST_REJECTED];
   st->ssa_steps = (int64_t)h[pst::ST_SSA];
   st->flagged = (int64_t)h[pst::ST_FLAGGED];
+  st->uniforms = (int64_t)h[pst::ST_UNIFORMS];
+  st->ptrs_iterations = (int64_t)h[pst::ST_PTRS];
+  st->inversion_steps = (int64_t)h[pst::ST_INV];
+  st->leap_decisions = (int64_t)h[pst::ST_DECISIONS];
+  st->sample_rows = (int64_t)h[pst::ST_SAMPLES];
   st->kernel_ms = (double)ms;
   return PST_OK;
 }

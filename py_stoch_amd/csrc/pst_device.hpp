@@ -56,75 +56,47 @@ __device__ __forceinline__ Words4 philox4x32_10(uint32_t c0, uint32_t c1, uint32
   return {c0, c1, c2, c3};
 }
 
-// Per-simulation stream: block b of simulation s is
-// Philox(ctr = (b_lo, b_hi, s_lo, s_hi), key = (seed_lo, seed_hi)).
-struct Stream {
-  uint32_t k0, k1;     // key (seed)
-  uint32_t s_lo, s_hi; // global simulation id
-  uint64_t blk;        // next block to generate
-  uint32_t c0, c1, c2; // cached words of the current block
-  int avail;           // cached words still unused (FP64: 0 or 2, FP32: 0..3)
+// Per-lane work counters feeding the work model of the roofline figure.
+struct Work {
+  unsigned long long uniforms;   // uniforms drawn
+  unsigned long long ptrs;       // PTRS rejection-loop iterations
+  unsigned long long inv;        // inversion search steps
+  unsigned long long decisions;  // tau-leap decisions (P1_P2 bodies with a0 > 0)
+  unsigned long long samples;    // sample rows written
+};
 
+// Per-simulation stream key.  Every draw comes from the Philox block
+//   Philox(ctr = (c_lo, c_hi | it << 16, s_lo, s_hi), key = (seed_lo, seed_hi))
+// for a per-simulation block counter c and a retry index it (non-zero only
+// for PTRS rejection iterations).  Blocks are taken in groups at fixed points
+// of the algorithm and every draw reads fixed word positions, so the whole
+// wavefront generates a group together instead of refilling lane by lane:
+//   SSA step            1 block  rho1 = U(w0,w1), rho2 = U(w2,w3)
+//   critical tau2/pick  1 block  rho1 = U(w0,w1), rho  = U(w2,w3)
+//   Poisson attempt     FP64: block b+j for reaction j, U = U(w0,w1),
+//                       V = U(w2,w3); mixed: block b + j/2, words 2(j%2),
+//                       2(j%2)+1
+struct Key {
+  uint32_t k0, k1, s_lo, s_hi;
   __device__ __forceinline__ void init(uint64_t seed, uint64_t sim) {
     k0 = (uint32_t)seed;
     k1 = (uint32_t)(seed >> 32);
     s_lo = (uint32_t)sim;
     s_hi = (uint32_t)(sim >> 32);
-    blk = 0;
-    avail = 0;
-    c0 = c1 = c2 = 0;
   }
-
-  __device__ __forceinline__ Words4 next_block() {
-    Words4 w = philox4x32_10((uint32_t)blk, (uint32_t)(blk >> 32), s_lo, s_hi, k0, k1);
-    blk += 1;
-    return w;
-  }
-
-  // FP64 uniform in (0, 1]: two words per draw, two draws per block.
-  //   m = (w_a >> 5) * 2^26 + (w_b >> 6),  u = (m + 1) * 2^-53
-  __device__ __forceinline__ double uniform_f64() {
-    uint32_t wa, wb;
-    if (avail) {
-      wa = c0;
-      wb = c1;
-      avail = 0;
-    } else {
-      Words4 w = next_block();
-      wa = w.w0;
-      wb = w.w1;
-      c0 = w.w2;
-      c1 = w.w3;
-      avail = 2;
-    }
-    const uint64_t m = ((uint64_t)(wa >> 5) << 26) | (uint64_t)(wb >> 6);
-    return (double)(m + 1u) * 0x1p-53;
-  }
-
-  // FP32 uniform in (0, 1]: one word per draw, four draws per block.
-  //   u = ((w >> 8) + 1) * 2^-24
-  __device__ __forceinline__ float uniform_f32() {
-    uint32_t w;
-    if (avail == 0) {
-      Words4 b = next_block();
-      w = b.w0;
-      c0 = b.w1;
-      c1 = b.w2;
-      c2 = b.w3;
-      avail = 3;
-    } else if (avail == 3) {
-      w = c0;
-      avail = 2;
-    } else if (avail == 2) {
-      w = c1;
-      avail = 1;
-    } else {
-      w = c2;
-      avail = 0;
-    }
-    return (float)((w >> 8) + 1u) * 0x1p-24f;
+  __device__ __forceinline__ Words4 block(uint64_t c, uint32_t it) const {
+    return philox4x32_10((uint32_t)c, (uint32_t)(c >> 32) | (it << 16), s_lo, s_hi, k0, k1);
   }
 };
+
+// FP64 uniform in (0, 1] from two words: m = (w_a >> 5) 2^26 + (w_b >> 6), (m + 1) 2^-53
+__device__ __forceinline__ double u53(uint32_t wa, uint32_t wb) {
+  const uint64_t m = ((uint64_t)(wa >> 5) << 26) | (uint64_t)(wb >> 6);
+  return (double)(m + 1u) * 0x1p-53;
+}
+
+// FP32 uniform in (0, 1] from one word: ((w >> 8) + 1) 2^-24
+__device__ __forceinline__ float u24(uint32_t w) { return (float)((w >> 8) + 1u) * 0x1p-24f; }
 
 // ---------------------------------------------------------------------------
 // FP64 log / exp / ln Gamma(k+1), bit-reproducible on host and device
@@ -218,42 +190,6 @@ __device__ __forceinline__ double lgamma_int_f64(double x) {  // x = k + 1
   return (x - 0.5) * log_f64(x) - x + 0.9189385332046727417803297 + ser;
 }
 
-// Poisson(lambda) in FP64; -1 when the PTRS iteration limit is hit.
-__device__ __forceinline__ int64_t poisson_f64(Stream& g, double lambda) {
-  if (!(lambda > 0.0)) return 0;
-  if (lambda < 10.0) {
-    const double u = g.uniform_f64();
-    double p = exp_f64(-lambda);
-    double F = p;
-    int64_t k = 0;
-    while (u > F && k < INV_MAX_K) {
-      k = k + 1;
-      p = p * lambda / (double)k;
-      F = F + p;
-    }
-    return k;
-  }
-  const double loglam = log_f64(lambda);
-  const double slam = exp_f64(0.5 * loglam);
-  const double b = 0.931 + 2.53 * slam;
-  const double a = -0.059 + 0.02483 * b;
-  const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
-  const double vr = 0.9277 - 3.6224 / (b - 2.0);
-  const double log_invalpha = log_f64(invalpha);
-  for (int it = 0; it < PTRS_MAX_ITERS; ++it) {
-    const double U = g.uniform_f64() - 0.5;
-    const double V = g.uniform_f64();
-    const double us = 0.5 - fabs(U);
-    const double kf = floor((2.0 * a / us + b) * U + lambda + 0.43);
-    if (us >= 0.07 && V <= vr) return (int64_t)kf;
-    if (kf < 0.0 || (us < 0.013 && V > us)) continue;
-    const double lhs = log_f64(V) + log_invalpha - log_f64(a / (us * us) + b);
-    const double rhs = -lambda + kf * loglam - lgamma_int_f64(kf + 1.0);
-    if (lhs <= rhs) return (int64_t)kf;
-  }
-  return -1;
-}
-
 // __float2int_rn (P1_P2.py:572): round half to even; NaN -> 0, clamped.
 __device__ __forceinline__ int32_t rn_f32_to_int(float f) {
   if (f != f) return 0;
@@ -268,44 +204,5 @@ __device__ __forceinline__ int32_t rn_f32_to_int(float f) {
 __device__ __forceinline__ float fast_log(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
 __device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-
-// Poisson(lambda) with the rejection-loop body in FP32; the rare squeeze-failure
-// acceptance test (about one draw in eight) is evaluated in FP64 so that the
-// log-density difference keeps full precision at large lambda.
-__device__ __forceinline__ int64_t poisson_mixed(Stream& g, float lambda) {
-  if (!(lambda > 0.0f)) return 0;
-  if (lambda < 10.0f) {
-    const float u = g.uniform_f32();
-    float p = fast_exp(-lambda);
-    float F = p;
-    int k = 0;
-    while (u > F && k < INV_MAX_K) {
-      k = k + 1;
-      p = p * lambda * fast_rcp((float)k);
-      F = F + p;
-    }
-    return k;
-  }
-  const float slam = __builtin_sqrtf(lambda);
-  const float b = 0.931f + 2.53f * slam;
-  const float a = -0.059f + 0.02483f * b;
-  const float invalpha = 1.1239f + 1.1328f * fast_rcp(b - 3.4f);
-  const float vr = 0.9277f - 3.6224f * fast_rcp(b - 2.0f);
-  const float twoa = 2.0f * a;
-  for (int it = 0; it < PTRS_MAX_ITERS; ++it) {
-    const float U = g.uniform_f32() - 0.5f;
-    const float V = g.uniform_f32();
-    const float us = 0.5f - __builtin_fabsf(U);
-    const float kf = __builtin_floorf((twoa * fast_rcp(us) + b) * U + lambda + 0.43f);
-    if (us >= 0.07f && V <= vr) return (int64_t)kf;
-    if (kf < 0.0f || (us < 0.013f && V > us)) continue;
-    const double lhs = (double)fast_log(V) + (double)fast_log(invalpha) -
-                       (double)fast_log(a * fast_rcp(us * us) + b);
-    const double dl = (double)lambda;
-    const double rhs = -dl + (double)kf * log_f64(dl) - lgamma_int_f64((double)kf + 1.0);
-    if (lhs <= rhs) return (int64_t)kf;
-  }
-  return -1;
-}
 
 }  // namespace pst

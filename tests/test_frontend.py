@@ -1,0 +1,216 @@
+"""Front end: SBML reader, SPN, TlParser / GParser, mass-action table,
+sim_maker sweeps, U round-up and the result writer.
+
+Expected values are derived by hand from the reference's rules:
+SPN.py:71-150 (Pre/Post/constant matrices, initial state, parameters,
+hazards), tl_parser.py:50-120 (V = Post - Pre, species-major flattening,
+H = highest reaction order over the reactions whose hazard names the species,
+H_type = largest Pre stoichiometry over those reactions), simulator.py:40-56
+(block thresholds and U round-up), sim_maker.py:46-64 (cartesian product).
+"""
+import os
+
+import numpy as np
+import pytest
+import sympy
+
+from py_stoch_amd import models, sim_maker
+from py_stoch_amd.mod.parser import TlArgs, TlParser, GArgs, GParser, Parser
+from py_stoch_amd.mod.parser import mass_action
+from py_stoch_amd.mod.parser.tl_parser import tables_of
+from py_stoch_amd.mod.petri_net import SPN, sbml
+from py_stoch_amd.mod.simulator.cuTauLeaping import CuTauLeaping
+from py_stoch_amd.mod.utils import save_results, Timer
+
+REF_EXAMPLES = "/root/reference/examples"
+needs_examples = pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="reference examples not mounted")
+
+
+def _rows(arr):
+    return [tuple(int(v) for v in r) for r in np.asarray(arr).reshape(-1, 4)]
+
+
+# ---------------------------------------------------------------- SBML + SPN
+@needs_examples
+def test_lotka_volterra_sbml():
+    spn = SPN.from_sbml_file(os.path.join(REF_EXAMPLES, "lotkaVoltera_sbml.xml"))
+    # species x, y, a; reactions a+x -> a+2x, x+y -> 2y, y -> 0
+    assert spn.P == {"species_0": 0, "species_1": 1, "species_2": 2}
+    assert list(spn.T.values()) == [0, 1, 2]
+    assert np.array_equal(spn.Pre, [[1, 1, 0], [0, 1, 1], [1, 0, 0]])
+    assert np.array_equal(spn.Post, [[2, 0, 0], [0, 2, 0], [1, 0, 0]])
+    assert spn.M == [1000.0, 1000.0, 1.0]
+    assert spn.c == [10.0, 0.01, 10.0]
+    S = sympy.symbols("species_0 species_1 species_2")
+    P = sympy.symbols("parameter_0 parameter_1 parameter_2")
+    expect = [P[0] * S[0] * S[2], P[1] * S[0] * S[1], P[2] * S[1]]
+    for h, e in zip(spn.h, expect):
+        assert sympy.simplify(sympy.sympify(h) - e) == 0
+    a = models.make_args(spn, E=[0, 1], t_max=10, ita=1001, U=1280)
+    assert _rows(a.A) == [(0, 0, 1, 0), (0, 1, 1, 0), (1, 1, 1, 0), (1, 2, 1, 0), (2, 0, 1, 0)]
+    assert _rows(a.V) == [(0, 0, 1, 0), (0, 1, -1, 0), (1, 1, 1, 0), (1, 2, -1, 0)]
+    assert a.A_size == 5 and a.V_size == 4
+    assert list(a.H) == [2, 2, 2] and list(a.H_type) == [1, 1, 1]
+    assert list(a.x_0) == [1000, 1000, 1]
+    assert a.I.dtype == np.float32 and a.I.shape == (1001,) and a.I[-1] == np.float32(10.0)
+    assert a.kappa == 2
+
+
+@needs_examples
+def test_schlogl_sbml_hors():
+    spn = SPN.from_sbml_file(os.path.join(REF_EXAMPLES, "schlogl_sbml.xml"))
+    a = models.make_args(spn)
+    # X: H = 3 (R1 has total degree 3 with a a species), H_type = 3 (R2 consumes 3 X)
+    # a: mentioned by R1 only (order 3), Pre 1; b: mentioned by R3 (order 1), Pre 1
+    assert list(a.H) == [3, 3, 1] and list(a.H_type) == [3, 1, 1]
+    assert _rows(a.V) == [(0, 0, 1, 0), (0, 1, -1, 0), (0, 2, 1, 0), (0, 3, -1, 0)]
+    assert np.array_equal(spn.StoichMConst[1:], np.ones((2, 4)))
+    assert np.array_equal(spn.StoichMConst[0], np.zeros(4))
+    assert np.array_equal(a.prop_order, [[2, 3, 0, 1], [1, 0, 0, 0], [0, 0, 1, 0]])
+    np.testing.assert_allclose(a.rate_fn(a.c), [1.5e-7, 1e-4 / 6, 1e-3, 3.5], rtol=1e-15)
+
+
+@needs_examples
+def test_schlogl_v2_parameters_as_species_constants():
+    spn = SPN.from_sbml_file(os.path.join(REF_EXAMPLES, "schlogl_sbmlv2.xml"))
+    a = models.make_args(spn)
+    assert a.N == 1 and a.M == 4
+    assert list(a.H) == [3] and list(a.H_type) == [3]
+    np.testing.assert_allclose(a.rate_fn(a.c), [3e-7 / 2 * 1e5, 1e-4 / 6, 1e-3 * 2e5, 3.5], rtol=1e-14)
+
+
+@needs_examples
+def test_simple_and_cdi_and_plasmid():
+    s = SPN.from_sbml_file(os.path.join(REF_EXAMPLES, "simple_sbml.xml"))
+    a = models.make_args(s)
+    assert list(a.x_0) == [10000, 1, 0]
+    # speciesReference without stoichiometry defaults to 1
+    assert _rows(a.V) == [(0, 0, -1, 0), (1, 0, 1, 0), (1, 1, -1, 0), (2, 1, 1, 0)]
+    assert list(a.H) == [1, 1, 0] and list(a.H_type) == [1, 1, 0]
+    c = models.make_args(SPN.from_sbml_file(os.path.join(REF_EXAMPLES, "cdi_sbml.xml")))
+    assert _rows(c.V) == [(0, 0, 1, 0), (1, 1, 1, 0), (1, 2, -1, 0)]
+    assert list(c.H) == [2, 2] and list(c.H_type) == [1, 1]
+    p = models.make_args(SPN.from_sbml_file(os.path.join(REF_EXAMPLES, "plasmid_stability.xml")))
+    # ((1-lambda)/Tp) Xp, (lambda(1-omega)/Tp) Xp, (lambda omega/Tp) Xp, Xm/Tm
+    np.testing.assert_allclose(p.rate_fn(p.c), [0.99, 0.01, 0.0, 1 / 0.75], rtol=1e-14)
+
+
+def test_sbml_reader_errors():
+    doc = sbml.SBMLReader().readSBMLFromString(b"<notsbml/>")
+    assert doc.getNumErrors() == 1 and doc.getModel() is None
+    doc = sbml.SBMLReader().readSBMLFromString(b"<sbml><model id='m'><listOfReactions><reaction id='r'>"
+                                               b"<kineticLaw><math xmlns='http://www.w3.org/1998/Math/MathML'>"
+                                               b"<apply><sin/><ci>x</ci></apply></math></kineticLaw></reaction>"
+                                               b"</listOfReactions></model></sbml>")
+    assert doc.getNumErrors() == 1
+
+
+# ---------------------------------------------------------- mass-action table
+def test_mass_action_forms():
+    h = ["parameter_0", "parameter_1*species_0", "parameter_2/2*species_0*(species_0 - 1)",
+         "parameter_3*species_0**2", "parameter_0*species_0*species_1*(species_1-1)*(species_1-2)/6"]
+    order, step, coeffs = mass_action.analyse(h, 2, 4)
+    assert np.array_equal(order, [[0, 1, 2, 2, 1], [0, 0, 0, 0, 3]])
+    assert step[0, 2] == 1 and step[0, 3] == 0 and step[1, 4] == 1
+    f = mass_action.RateFunction(coeffs, 4)
+    np.testing.assert_allclose(f([1.0, 2.0, 3.0, 4.0]), [1.0, 2.0, 1.5, 4.0, 1.0 / 6.0])
+
+
+@pytest.mark.parametrize("haz", ["parameter_0*species_0/(1+species_0)", "species_0 + species_1",
+                                 "parameter_0*species_0*(species_0-2)", "exp(species_0)"])
+def test_mass_action_rejects(haz):
+    with pytest.raises(mass_action.UnsupportedHazard):
+        mass_action.analyse([haz], 2, 1)
+
+
+def test_mass_action_evaluates_like_the_hazard():
+    rng = np.random.default_rng(0)
+    spn = models.random_network(rng, n_species=6, n_reactions=20)
+    a = models.make_args(spn, E=[0])
+    k = a.rate_fn(a.c)
+    S = sympy.symbols(" ".join("species_%d" % i for i in range(a.N)))
+    Pm = sympy.symbols(" ".join("parameter_%d" % i for i in range(len(a.c))))
+    for trial in range(5):
+        x = rng.integers(0, 20, size=a.N)
+        for j, h in enumerate(spn.h):
+            ref = float(sympy.sympify(h).subs(dict(zip(S, x))).subs(dict(zip(Pm, a.c))))
+            got = k[j]
+            for i in range(a.N):
+                for q in range(a.prop_order[i, j]):
+                    got *= x[i] - a.prop_step[i, j] * q
+            assert got == pytest.approx(ref, rel=1e-12, abs=1e-300)
+
+
+# ------------------------------------------------------------- run settings
+def test_calculate_sizes_round_up():
+    sim = CuTauLeaping()
+    for U, exp_block, exp_U in ((1, 1, 1), (2, 2, 2), (3, 32, 32), (100, 32, 128), (640, 64, 640), (641, 64, 704),
+                                (1280, 128, 1280), (2000, 128, 2048), (2560, 256, 2560), (65536, 256, 65536),
+                                (65537, 256, 65792)):
+        a = TlArgs()
+        a.U = U
+        grid, block = sim.calculate_sizes(a)
+        assert block == exp_block and a.U == exp_U and grid * block == exp_U, U
+
+
+def test_gparser_and_settings(tmp_path):
+    g = GParser.parse(GArgs(), models.birth_death())
+    assert g.N == 1 and g.M == 2 and _rows(g.V) == [(0, 0, 1, 0), (0, 1, -1, 0)]
+    f = tmp_path / "settings.xml"
+    f.write_text("<settings><output_species><species idx='0'/><species idx='2'/></output_species>"
+                 "<duration>5.5</duration><num_output_time_points>12</num_output_time_points>"
+                 "<num_simulations>100</num_simulations></settings>")
+    t = TlArgs()
+    Parser.parse_settings(str(f), t)
+    assert list(t.E) == [0, 2] and t.t_max == 5.5 and t.ita == 12 and t.U == 100
+
+
+def test_define_hazards_codegen_text():
+    code = Parser.define_hazards(models.lotka_volterra())
+    assert "a[0] = d_c[0]*x[0];" in code and "a[1] = d_c[1]*x[0]*x[1];" in code
+
+
+def test_flatten_matrix_widened():
+    m = np.zeros((200, 300))
+    m[150, 250] = -3
+    m[199, 0] = 2
+    rows, n = Parser.flatten_matrix(m)
+    assert n == 2 and _rows(rows) == [(150, 250, -3, 0), (199, 0, 2, 0)]
+
+
+# ---------------------------------------------------------------- sim_maker
+def test_rest_list_cartesian_product():
+    assert sim_maker.rest_list([[1, 2], [3], [4, 5]]) == [[1, 3, 4], [1, 3, 5], [2, 3, 4], [2, 3, 5]]
+    assert sim_maker.rest_list([[7, 8]]) == [7, 8]
+
+
+@needs_examples
+def test_make_list_sweep():
+    model = sbml.read_model(os.path.join(REF_EXAMPLES, "lotkaVoltera_sbml.xml"))
+    lst = sim_maker.TLArgsList.make_list(model, [[1000, 500], [1000], [1]], [[10], [0.01, 0.02], [10]],
+                                         [0, 1], 10, 101, 1280, gillespie=1)
+    assert len(lst) == 4
+    assert [list(a.x_0) for a in lst] == [[1000, 1000, 1]] * 2 + [[500, 1000, 1]] * 2
+    assert [list(a.c) for a in lst] == [[10, 0.01, 10], [10, 0.02, 10]] * 2
+    assert all(a.gillespie == 1 and a.U == 1280 and a.ita == 101 for a in lst)
+    np.testing.assert_allclose(lst[1].rate_fn(lst[1].c), [10, 0.02, 10])
+
+
+# -------------------------------------------------------------------- utils
+def test_save_results_format(tmp_path):
+    a = models.make_args(models.lotka_volterra(), E=[0, 1], t_max=1.0, ita=3, U=4)
+    O = np.arange(2 * 3 * 4, dtype=np.int32).reshape(2, 3, 4)
+    files = save_results(O, a, "lv", sim_type="TL", out_root=str(tmp_path))
+    assert len(files) == 2
+    assert os.path.dirname(files[0]).endswith(os.path.join("lv", "x-1000-1000_c-1-0.01-1"))
+    body = np.loadtxt(files[1], dtype=np.int64)
+    assert np.array_equal(body, O[1])
+    with open(files[0]) as fh:
+        head = fh.read()
+    assert "num_simulations: 4" in head and "duration: 1.0" in head
+
+
+def test_timer():
+    with Timer() as t:
+        sum(range(1000))
+    assert t.msecs >= 0.0
