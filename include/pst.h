@@ -18,6 +18,10 @@
  *                        the trajectory array left in device memory (no
  *                        reference counterpart: the reference allocates,
  *                        uploads and frees per run, cuTauLeaping.py:38-43,83)
+ *   pst_batch_*          a list of TlArgs-like jobs (a sim_maker sweep,
+ *                        sim_maker.py:8-43, or several networks) in one
+ *                        persistent launch; the reference runs them one by
+ *                        one (examples/simple_sim.py:65-77)
  *
  * Table fields mirror TlArgs (mod/parser/tl_parser.py:8-47): A and V are the
  * reference's flattened char4 tables (parser.py:58-71) widened to int32 x 4.
@@ -108,6 +112,15 @@ typedef struct pst_stats {
 
 typedef struct pst_plan pst_plan;
 
+/* one job of a batch: a network, its run settings and its parameter rows */
+typedef struct pst_job {
+  pst_network net;
+  pst_config cfg;             /* cfg.precision must be equal for all jobs  */
+  const double* rates;        /* [n_sets][M]                                */
+  const int32_t* x0;          /* [n_sets][N]                                */
+  int32_t n_sets;
+} pst_job;
+
 /*
  * Run cfg->n_sims simulations and write the trajectories into the host array
  * O[kappa][ita][n_sims] (int32, simulation fastest, the layout of
@@ -143,6 +156,30 @@ int pst_plan_execute(pst_plan* plan, int32_t* O_dev, int64_t o_stride, int32_t* 
                      void* stream);
 int pst_plan_read_stats(pst_plan* plan, pst_stats* stats);
 void pst_plan_destroy(pst_plan* plan);
+
+/*
+ * Re-key a plan's random stream for its next executions: simulation s then
+ * draws from (seed, sim_offset + s).  Runs with disjoint id ranges are
+ * independent samples of the same workload.
+ */
+int pst_plan_set_stream(pst_plan* plan, uint64_t seed, uint64_t sim_offset);
+
+/*
+ * Batches: many jobs -- parameter sweeps (rows of rates/x0), several networks
+ * of uneven size and step count -- in one persistent launch per size class.
+ * Wavefronts take simulations from a device work queue per job, so lanes whose
+ * simulation ends early pick up the next one.  No reference counterpart: the
+ * reference runs the TlArgs of a sweep one after the other
+ * (examples/simple_sim.py:65-77, sim_maker.py:8-43).
+ * pst_batch_run: host arrays O[q] = [kappa_q][ita_q][n_sims_q], flags[q]
+ * (the list and its entries nullable).  pst_batch_plan_execute: device arrays
+ * with o_strides[q] >= n_sims_q.
+ */
+int pst_batch_run(const pst_job* jobs, int32_t n_jobs, int32_t* const* O, int32_t* const* flags,
+                  pst_stats* stats);
+int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** plan);
+int pst_batch_plan_execute(pst_plan* plan, int32_t* const* O_dev, const int64_t* o_strides,
+                           int32_t* const* flags_dev, void* stream);
 
 /* largest network the compiled kernels accept (species, reactions) */
 void pst_max_network(int32_t* max_species, int32_t* max_reactions);

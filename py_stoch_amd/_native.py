@@ -85,13 +85,27 @@ class PstStats(ctypes.Structure):
         return d
 
 
+class PstJob(ctypes.Structure):
+    _fields_ = [
+        ("net", PstNetwork),
+        ("cfg", PstConfig),
+        ("rates", _P_F64),
+        ("x0", _P_I32),
+        ("n_sets", ctypes.c_int32),
+    ]
+
+
 EXPORTED_SYMBOLS = (
     "pst_tau_leaping_run",
     "pst_gillespie_run",
+    "pst_batch_run",
+    "pst_batch_plan_create",
+    "pst_batch_plan_execute",
     "pst_plan_create",
     "pst_plan_execute",
     "pst_plan_read_stats",
     "pst_plan_destroy",
+    "pst_plan_set_stream",
     "pst_max_network",
     "pst_last_error",
     "pst_abi_version",
@@ -120,6 +134,15 @@ def load():
     L.pst_tau_leaping_run.restype = ctypes.c_int
     L.pst_gillespie_run.argtypes = run_args
     L.pst_gillespie_run.restype = ctypes.c_int
+    L.pst_batch_run.argtypes = [ctypes.POINTER(PstJob), ctypes.c_int32, ctypes.POINTER(_P_I32),
+                                ctypes.POINTER(_P_I32), ctypes.POINTER(PstStats)]
+    L.pst_batch_run.restype = ctypes.c_int
+    L.pst_batch_plan_create.argtypes = [ctypes.POINTER(PstJob), ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+    L.pst_batch_plan_create.restype = ctypes.c_int
+    L.pst_batch_plan_execute.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_void_p),
+                                         ctypes.c_void_p]
+    L.pst_batch_plan_execute.restype = ctypes.c_int
     L.pst_plan_create.argtypes = [ctypes.POINTER(PstNetwork), ctypes.POINTER(PstConfig), _P_F64,
                                   _P_I32, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
     L.pst_plan_create.restype = ctypes.c_int
@@ -128,6 +151,8 @@ def load():
     L.pst_plan_execute.restype = ctypes.c_int
     L.pst_plan_read_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(PstStats)]
     L.pst_plan_read_stats.restype = ctypes.c_int
+    L.pst_plan_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64]
+    L.pst_plan_set_stream.restype = ctypes.c_int
     L.pst_plan_destroy.argtypes = [ctypes.c_void_p]
     L.pst_plan_destroy.restype = None
     L.pst_max_network.argtypes = [_P_I32, _P_I32]
@@ -226,6 +251,89 @@ class Plan:
                                      ctypes.c_void_p(int(flags_dev_ptr)) if flags_dev_ptr else None,
                                      ctypes.c_void_p(int(stream_ptr)) if stream_ptr else None)
         _check(rc, "pst_plan_execute")
+
+    def set_stream(self, seed, sim_offset=0):
+        """Re-key the random stream of the following executions."""
+        _check(load().pst_plan_set_stream(self._h, int(seed), int(sim_offset)), "pst_plan_set_stream")
+
+    def stats(self):
+        st = PstStats()
+        _check(load().pst_plan_read_stats(self._h, ctypes.byref(st)), "pst_plan_read_stats")
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().pst_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Job:
+    """One job of a batch: tables, parameter rows and run settings (the
+    arguments of :func:`simulate`)."""
+
+    def __init__(self, tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64, n_c=10,
+                 eta=0.03, ssa_burst=100, t_max=1.0, I=None, E=(0,), n_sims=1, sims_per_set=None, seed=0,
+                 sim_offset=0):
+        self.m = _Marshalled(tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I, E, n_sims,
+                             sims_per_set, seed, sim_offset)
+        self.n_sims = int(n_sims)
+        self.kappa = self.m.E.shape[0]
+        self.ita = self.m.I.shape[0]
+
+    def struct(self):
+        return PstJob(self.m.net, self.m.cfg, _ptr(self.m.rates, ctypes.c_double),
+                      _ptr(self.m.x0, ctypes.c_int32), self.m.n_sets)
+
+
+def _job_array(jobs):
+    arr = (PstJob * len(jobs))()
+    for i, j in enumerate(jobs):
+        arr[i] = j.struct()
+    return arr
+
+
+def batch_simulate(jobs):
+    """Run several jobs in one persistent launch per size class; returns
+    ([O_q], [flags_q], stats)."""
+    L = load()
+    arr = _job_array(jobs)
+    Os = [np.empty((j.kappa, j.ita, j.n_sims), np.int32) for j in jobs]
+    Fs = [np.zeros(max(j.n_sims, 1), np.int32) for j in jobs]
+    O_ptrs = (_P_I32 * len(jobs))(*[_ptr(o, ctypes.c_int32) for o in Os])
+    F_ptrs = (_P_I32 * len(jobs))(*[_ptr(f, ctypes.c_int32) for f in Fs])
+    st = PstStats()
+    _check(L.pst_batch_run(arr, len(jobs), O_ptrs, F_ptrs, ctypes.byref(st)), "pst_batch_run")
+    return Os, [f[: j.n_sims] for f, j in zip(Fs, jobs)], st.as_dict()
+
+
+class BatchPlan:
+    """Device-resident batch: tables uploaded once; execute() takes device
+    addresses of one trajectory array (and optionally one flag array) per job."""
+
+    def __init__(self, jobs):
+        self.jobs = list(jobs)
+        h = ctypes.c_void_p()
+        self._arr = _job_array(self.jobs)
+        _check(load().pst_batch_plan_create(self._arr, len(self.jobs), ctypes.byref(h)), "pst_batch_plan_create")
+        self._h = h
+
+    def execute(self, O_ptrs, strides, flag_ptrs=None, stream_ptr=None):
+        n = len(self.jobs)
+        Op = (ctypes.c_void_p * n)(*[int(p) for p in O_ptrs])
+        Sp = (ctypes.c_int64 * n)(*[int(s) for s in strides])
+        Fp = (ctypes.c_void_p * n)(*[int(p) if p else None for p in flag_ptrs]) if flag_ptrs else None
+        _check(load().pst_batch_plan_execute(self._h, Op, Sp, Fp,
+                                             ctypes.c_void_p(int(stream_ptr)) if stream_ptr else None),
+               "pst_batch_plan_execute")
+
+    def set_stream(self, seed, sim_offset=0):
+        _check(load().pst_plan_set_stream(self._h, int(seed), int(sim_offset)), "pst_plan_set_stream")
 
     def stats(self):
         st = PstStats()

@@ -1,7 +1,7 @@
 // pst_args.hpp -- kernel argument blocks shared by the launch code and the kernels.
 //
 // A network is passed to the kernels as a dense, zero-padded species x reaction
-// table (NetTab) by value in the kernarg segment: every entry is the same for
+// table (NetTab) inside a job descriptor in device memory: every entry is the same for
 // all lanes of a launch, so the compiler reads it with scalar loads into SGPRs
 // and every branch on it is a scalar branch.  Per-simulation data (state, time,
 // sample cursor, Philox counter) lives in VGPRs for the whole run.
@@ -24,7 +24,14 @@ struct NetTab {
   int8_t E[NP];         // recorded species (d_E, P1_P2.py:13)
 };
 
-struct RunArgs {
+enum StatIdx {
+  ST_EVENTS = 0, ST_LEAPS, ST_REJECTED, ST_SSA, ST_FLAGGED,
+  ST_UNIFORMS, ST_PTRS, ST_INV, ST_DECISIONS, ST_SAMPLES,
+  ST_COUNT = 16
+};
+
+// Settings of one job of a launch (uniform across the wavefronts working on it)
+struct JobParams {
   int32_t N, M;
   int32_t kappa, ita;
   int32_t n_c;          // critical threshold (tl_parser.py:43)
@@ -32,9 +39,7 @@ struct RunArgs {
   int32_t method;       // 0 tau-leaping, 1 forced SSA, 2 pure SSA
   int32_t n_sims;
   int32_t sims_per_set; // simulations sharing one rates/x0 row
-  int32_t precision;    // 0 fp64, 1 mixed
   int32_t ssa_slice;    // SSA steps per pass while other lanes wait to decide (launch tuning)
-  int32_t pad1;
   double eta;           // tl_parser.py:44
   double t_max;
   uint64_t seed;
@@ -45,13 +50,69 @@ struct RunArgs {
   const float* I;       // [ita] sample times
   int32_t* O;           // [kappa][ita][o_stride]
   int32_t* flags;       // [n_sims] or null
-  unsigned long long* stats;  // ST_COUNT counters, see StatIdx
 };
 
-enum StatIdx {
-  ST_EVENTS = 0, ST_LEAPS, ST_REJECTED, ST_SSA, ST_FLAGGED,
-  ST_UNIFORMS, ST_PTRS, ST_INV, ST_DECISIONS, ST_SAMPLES,
-  ST_COUNT = 16
+template <int NP, int MP>
+struct JobDev {
+  NetTab<NP, MP> tab;
+  JobParams p;
+};
+
+// Kernel-side view of the current job.  The table bytes are copied four to a
+// 32-bit word into SGPRs and hidden from the optimizer at the top of every
+// pass of the simulation loop (refresh(): an empty asm on a VGPR copy, then
+// readfirstlane), so the per-use byte extracts stay cheap scalar bit-field
+// extracts inside the loop instead of ~64 hoisted, long-lived SGPRs that
+// spill.  Scalar loads inside the loop are avoided: with one wavefront per
+// SIMD their latency is not hidden.
+template <int NP, int MP>
+struct JobRef {
+  static constexpr int NW = (NP * MP + 3) / 4;
+  static constexpr int NS = (NP + 3) / 4;
+  uint32_t v[NW], ord[NW], stp[NW];
+  uint32_t H[NS], Ht[NS], irs[NS], E[NS];
+  JobParams P;
+
+  __device__ __forceinline__ void load(const JobDev<NP, MP>* jd) {
+    __builtin_memcpy(v, jd->tab.v, sizeof(jd->tab.v));
+    __builtin_memcpy(ord, jd->tab.ord, sizeof(jd->tab.ord));
+    __builtin_memcpy(stp, jd->tab.stp, sizeof(jd->tab.stp));
+    __builtin_memcpy(H, jd->tab.H, sizeof(jd->tab.H));
+    __builtin_memcpy(Ht, jd->tab.Ht, sizeof(jd->tab.Ht));
+    __builtin_memcpy(irs, jd->tab.irs, sizeof(jd->tab.irs));
+    __builtin_memcpy(E, jd->tab.E, sizeof(jd->tab.E));
+    P = jd->p;
+  }
+  static __device__ __forceinline__ uint32_t opaque(uint32_t w) {
+    asm volatile("" : "+v"(w));
+    return __builtin_amdgcn_readfirstlane(w);
+  }
+  __device__ __forceinline__ void refresh() {
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      v[w] = opaque(v[w]);
+      ord[w] = opaque(ord[w]);
+      stp[w] = opaque(stp[w]);
+    }
+#pragma unroll
+    for (int w = 0; w < NS; ++w) {
+      H[w] = opaque(H[w]);
+      Ht[w] = opaque(Ht[w]);
+      irs[w] = opaque(irs[w]);
+      E[w] = opaque(E[w]);
+    }
+  }
+  static __device__ __forceinline__ int sbyte(const uint32_t* w, int idx) {
+    return (int)(int8_t)(w[idx >> 2] >> (8 * (idx & 3)));
+  }
+  __device__ __forceinline__ const JobParams& p() const { return P; }
+  __device__ __forceinline__ int V(int i, int j) const { return sbyte(v, i * MP + j); }
+  __device__ __forceinline__ int Ord(int i, int j) const { return sbyte(ord, i * MP + j); }
+  __device__ __forceinline__ int Stp(int i, int j) const { return sbyte(stp, i * MP + j); }
+  __device__ __forceinline__ int Hv(int i) const { return sbyte(H, i); }
+  __device__ __forceinline__ int Htv(int i) const { return sbyte(Ht, i); }
+  __device__ __forceinline__ int Irs(int i) const { return sbyte(irs, i); }
+  __device__ __forceinline__ int Ev(int q) const { return sbyte(E, q); }
 };
 
 // host-side copy of a network in dense form, built by the C-ABI layer
@@ -88,62 +149,22 @@ inline NetTab<NP, MP> make_tab(const HostNet& h) {
   return t;
 }
 
-// Kernel-side view of a NetTab: the table bytes packed four to a 32-bit word
-// and made opaque to the optimizer at the top of each simulation-loop
-// iteration (refresh()), so that the per-use byte extractions stay inside the
-// loop as cheap scalar bit-field extracts instead of being hoisted into one
-// SGPR per table entry (which spills).
-template <int NP, int MP>
-struct TabView {
-  static constexpr int NW = (NP * MP + 3) / 4;
-  static constexpr int NS = (NP + 3) / 4;
-  uint32_t v[NW], ord[NW], stp[NW];
-  uint32_t H[NS], Ht[NS], irs[NS], E[NS];
+// Launchers, one per translation unit (precision).  `jobs` is a device array
+// of JobDev<NP,MP> for the size class `cls`, `queue` n_jobs zeroed counters,
+// `waves` the number of wavefronts to launch.
+int launch_fp64(int cls, const void* jobs, int n_jobs, unsigned int* queue, unsigned long long* stats,
+                int waves, hipStream_t stream);
+int launch_mixed(int cls, const void* jobs, int n_jobs, unsigned int* queue, unsigned long long* stats,
+                 int waves, hipStream_t stream);
+// resident 256-lane blocks per CU for a class (occupancy)
+int blocks_per_cu_fp64(int cls);
+int blocks_per_cu_mixed(int cls);
 
-  __device__ __forceinline__ void load(const NetTab<NP, MP>& t) {
-    __builtin_memcpy(v, t.v, sizeof(t.v));
-    __builtin_memcpy(ord, t.ord, sizeof(t.ord));
-    __builtin_memcpy(stp, t.stp, sizeof(t.stp));
-    __builtin_memcpy(H, t.H, sizeof(t.H));
-    __builtin_memcpy(Ht, t.Ht, sizeof(t.Ht));
-    __builtin_memcpy(irs, t.irs, sizeof(t.irs));
-    __builtin_memcpy(E, t.E, sizeof(t.E));
-  }
-  // An empty asm with a VGPR operand hides the value from the optimizer;
-  // readfirstlane brings it back to an SGPR (all lanes hold the same word).
-  static __device__ __forceinline__ uint32_t opaque(uint32_t w) {
-    asm volatile("" : "+v"(w));
-    return __builtin_amdgcn_readfirstlane(w);
-  }
-  __device__ __forceinline__ void refresh() {
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      v[w] = opaque(v[w]);
-      ord[w] = opaque(ord[w]);
-      stp[w] = opaque(stp[w]);
-    }
-#pragma unroll
-    for (int w = 0; w < NS; ++w) {
-      H[w] = opaque(H[w]);
-      Ht[w] = opaque(Ht[w]);
-      irs[w] = opaque(irs[w]);
-      E[w] = opaque(E[w]);
-    }
-  }
-  static __device__ __forceinline__ int sbyte(const uint32_t* w, int idx) {
-    return (int)(int8_t)(w[idx >> 2] >> (8 * (idx & 3)));
-  }
-  __device__ __forceinline__ int V(int i, int j) const { return sbyte(v, i * MP + j); }
-  __device__ __forceinline__ int Ord(int i, int j) const { return sbyte(ord, i * MP + j); }
-  __device__ __forceinline__ int Stp(int i, int j) const { return sbyte(stp, i * MP + j); }
-  __device__ __forceinline__ int Hv(int i) const { return sbyte(H, i); }
-  __device__ __forceinline__ int Htv(int i) const { return sbyte(Ht, i); }
-  __device__ __forceinline__ int Irs(int i) const { return sbyte(irs, i); }
-  __device__ __forceinline__ int Ev(int q) const { return sbyte(E, q); }
-};
-
-// launchers, one per translation unit (precision)
-int launch_fp64(const HostNet& net, const RunArgs& ra, hipStream_t stream);
-int launch_mixed(const HostNet& net, const RunArgs& ra, hipStream_t stream);
+// smallest compiled size class holding (N, M, kappa), or -1
+inline int size_class(int N, int M, int kappa) {
+  for (int c = 0; c < NUM_CLASSES; ++c)
+    if (N <= CLASS_NP[c] && M <= CLASS_MP[c] && kappa <= CLASS_NP[c]) return c;
+  return -1;
+}
 
 }  // namespace pst

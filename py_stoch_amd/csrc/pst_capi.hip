@@ -3,15 +3,17 @@
 // Validation, table densification, device allocation and launch.  The
 // reference's host driver (cuTauLeaping.py:18-84) JIT-compiles two kernels per
 // run, uploads constants, launches P1_P2 and P3 once per iteration with a
-// gpuarray.sum and a scalar copy back each time; here one launch runs every
-// simulation to the end and only the trajectories (and 8 counters) come back.
+// gpuarray.sum and a scalar copy back each time; here one persistent launch
+// per size class runs every simulation of every job to the end and only the
+// trajectories (and 16 counters) come back.
 #include <hip/hip_runtime.h>
 
-#include <cstdio>
-#include <cstdarg>
-#include <cstring>
 #include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
 #include <new>
+#include <vector>
 
 #include "../../include/pst.h"
 #include "pst_args.hpp"
@@ -34,6 +36,12 @@ int hip_fail(hipError_t e, const char* what) {
 
 constexpr int kMaxSpecies = 16;
 constexpr int kMaxReactions = 16;
+constexpr int kSsaSlice = 4;
+
+__global__ void zero_u32(unsigned int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
 
 __global__ void zero_u64(unsigned long long* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -119,22 +127,7 @@ int prepare(const pst_network* net, const pst_config* cfg, const double* rates, 
   return PST_OK;
 }
 
-}  // namespace
-
-struct pst_plan {
-  pst::HostNet net;
-  pst::RunArgs ra;
-  int device;
-  int n_sets;
-  double* d_rates = nullptr;
-  int32_t* d_x0 = nullptr;
-  float* d_I = nullptr;
-  unsigned long long* d_stats = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool executed = false;
-};
-
-static int check_device() {
+int check_device(int* cu_count) {
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
   if (e != hipSuccess || n == 0) return fail(PST_ENODEV, "no HIP device visible");
@@ -144,8 +137,51 @@ static int check_device() {
   if ((e = hipGetDeviceProperties(&p, dev)) != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
   if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0)
     return fail(PST_ENODEV, "device %d is %s; this library is built for gfx950 only", dev, p.gcnArchName);
+  *cu_count = p.multiProcessorCount;
   return PST_OK;
 }
+
+// one job of a plan: validated tables on the host, parameters on the device
+struct PlanJob {
+  pst::HostNet net;
+  pst::JobParams p;  // O / o_stride / flags filled at execute time
+  int cls = -1;
+  double* d_rates = nullptr;
+  int32_t* d_x0 = nullptr;
+  float* d_I = nullptr;
+};
+
+template <int NP, int MP>
+void pack_jobs(const std::vector<PlanJob>& jobs, int cls, std::vector<unsigned char>& out, int* count) {
+  std::vector<pst::JobDev<NP, MP>> v;
+  for (const PlanJob& j : jobs) {
+    if (j.cls != cls) continue;
+    pst::JobDev<NP, MP> d;
+    std::memset(&d, 0, sizeof(d));
+    d.tab = pst::make_tab<NP, MP>(j.net);
+    d.p = j.p;
+    v.push_back(d);
+  }
+  out.resize(v.size() * sizeof(pst::JobDev<NP, MP>));
+  if (!v.empty()) std::memcpy(out.data(), v.data(), out.size());
+  *count = (int)v.size();
+}
+
+}  // namespace
+
+struct pst_plan {
+  int device = -1;
+  int precision = 0;
+  int cu_count = 0;
+  std::vector<PlanJob> jobs;
+  void* d_jobs[pst::NUM_CLASSES] = {nullptr, nullptr, nullptr};
+  size_t d_jobs_bytes[pst::NUM_CLASSES] = {0, 0, 0};
+  std::vector<unsigned char> uploaded[pst::NUM_CLASSES];  // last descriptors copied to d_jobs
+  unsigned int* d_queue = nullptr;
+  unsigned long long* d_stats = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool executed = false;
+};
 
 extern "C" {
 
@@ -160,92 +196,197 @@ void pst_max_network(int32_t* max_species, int32_t* max_reactions) {
 
 void pst_plan_destroy(pst_plan* p) {
   if (!p) return;
-  if (p->d_rates) (void)hipFree(p->d_rates);
-  if (p->d_x0) (void)hipFree(p->d_x0);
-  if (p->d_I) (void)hipFree(p->d_I);
+  for (PlanJob& j : p->jobs) {
+    if (j.d_rates) (void)hipFree(j.d_rates);
+    if (j.d_x0) (void)hipFree(j.d_x0);
+    if (j.d_I) (void)hipFree(j.d_I);
+  }
+  for (int c = 0; c < pst::NUM_CLASSES; ++c)
+    if (p->d_jobs[c]) (void)hipFree(p->d_jobs[c]);
+  if (p->d_queue) (void)hipFree(p->d_queue);
   if (p->d_stats) (void)hipFree(p->d_stats);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   delete p;
 }
 
-int pst_plan_create(const pst_network* net, const pst_config* cfg, const double* rates, const int32_t* x0,
-                    int32_t n_sets, pst_plan** out) {
+int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
   g_err[0] = 0;
   if (!out) return fail(PST_EINVAL, "null plan pointer");
   *out = nullptr;
+  if (!jobs || n_jobs < 1) return fail(PST_EINVAL, "need at least one job");
+  // validate every job on the host before touching the device
+  std::vector<pst::HostNet> nets(n_jobs);
+  for (int q = 0; q < n_jobs; ++q) {
+    const pst_job& J = jobs[q];
+    int rc = prepare(&J.net, &J.cfg, J.rates, J.x0, J.n_sets, &nets[q]);
+    if (rc != PST_OK) {
+      if (n_jobs > 1) {
+        char msg[512];
+        std::snprintf(msg, sizeof(msg), "job %d: %s", q, g_err);
+        std::snprintf(g_err, sizeof(g_err), "%s", msg);
+      }
+      return rc;
+    }
+    if (J.cfg.precision != jobs[0].cfg.precision)
+      return fail(PST_EINVAL, "job %d has precision %d, job 0 has %d: one precision per batch", q,
+                  J.cfg.precision, jobs[0].cfg.precision);
+    if (pst::size_class(nets[q].N, nets[q].M, nets[q].kappa) < 0)
+      return fail(PST_ETOOLARGE, "job %d: no compiled kernel for N=%d, M=%d, kappa=%d", q, nets[q].N, nets[q].M,
+                  nets[q].kappa);
+  }
+  int cu = 0;
+  int rc = check_device(&cu);
+  if (rc != PST_OK) return rc;
   pst_plan* p = new (std::nothrow) pst_plan();
   if (!p) return fail(PST_ENOMEM, "host allocation failed");
-  int rc = prepare(net, cfg, rates, x0, n_sets, &p->net);
-  if (rc != PST_OK) { delete p; return rc; }
-  if ((rc = check_device()) != PST_OK) { delete p; return rc; }
   (void)hipGetDevice(&p->device);
-  const int N = net->n_species, M = net->n_reactions;
-  const int64_t sets = cfg->n_sims == 0 ? 1 : ((int64_t)cfg->n_sims + cfg->sims_per_set - 1) / cfg->sims_per_set;
-  p->n_sets = (int)sets;
+  p->cu_count = cu;
+  p->precision = jobs[0].cfg.precision;
+  p->jobs.resize(n_jobs);
   hipError_t e;
-  if ((e = hipMalloc(&p->d_rates, sizeof(double) * sets * M)) != hipSuccess) { pst_plan_destroy(p); return hip_fail(e, "hipMalloc(rates)"); }
-  if ((e = hipMalloc(&p->d_x0, sizeof(int32_t) * sets * N)) != hipSuccess) { pst_plan_destroy(p); return hip_fail(e, "hipMalloc(x0)"); }
-  if ((e = hipMalloc(&p->d_I, sizeof(float) * cfg->ita)) != hipSuccess) { pst_plan_destroy(p); return hip_fail(e, "hipMalloc(I)"); }
-  if ((e = hipMalloc(&p->d_stats, sizeof(unsigned long long) * pst::ST_COUNT)) != hipSuccess) { pst_plan_destroy(p); return hip_fail(e, "hipMalloc(stats)"); }
-  if ((e = hipMemcpy(p->d_rates, rates, sizeof(double) * sets * M, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(p->d_x0, x0, sizeof(int32_t) * sets * N, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(p->d_I, cfg->I, sizeof(float) * cfg->ita, hipMemcpyHostToDevice)) != hipSuccess) {
+  for (int q = 0; q < n_jobs; ++q) {
+    const pst_job& J = jobs[q];
+    PlanJob& pj = p->jobs[q];
+    pj.net = nets[q];
+    pj.cls = pst::size_class(pj.net.N, pj.net.M, pj.net.kappa);
+    const int N = J.net.n_species, M = J.net.n_reactions;
+    const int64_t sets = J.cfg.n_sims == 0 ? 1 : ((int64_t)J.cfg.n_sims + J.cfg.sims_per_set - 1) / J.cfg.sims_per_set;
+    if ((e = hipMalloc(&pj.d_rates, sizeof(double) * sets * M)) != hipSuccess ||
+        (e = hipMalloc(&pj.d_x0, sizeof(int32_t) * sets * N)) != hipSuccess ||
+        (e = hipMalloc(&pj.d_I, sizeof(float) * J.cfg.ita)) != hipSuccess) {
+      pst_plan_destroy(p);
+      return hip_fail(e, "hipMalloc(job tables)");
+    }
+    if ((e = hipMemcpy(pj.d_rates, J.rates, sizeof(double) * sets * M, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(pj.d_x0, J.x0, sizeof(int32_t) * sets * N, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(pj.d_I, J.cfg.I, sizeof(float) * J.cfg.ita, hipMemcpyHostToDevice)) != hipSuccess) {
+      pst_plan_destroy(p);
+      return hip_fail(e, "hipMemcpy(job tables)");
+    }
+    pst::JobParams& P = pj.p;
+    std::memset(&P, 0, sizeof(P));
+    P.N = N;
+    P.M = M;
+    P.kappa = J.cfg.kappa;
+    P.ita = J.cfg.ita;
+    P.n_c = J.cfg.n_c;
+    P.ssa_burst = J.cfg.ssa_burst;
+    P.method = J.cfg.method;
+    P.n_sims = J.cfg.n_sims;
+    P.sims_per_set = J.cfg.sims_per_set;
+    P.ssa_slice = kSsaSlice;
+    P.eta = J.cfg.eta;
+    P.t_max = J.cfg.t_max;
+    P.seed = J.cfg.seed;
+    P.sim_offset = J.cfg.sim_offset;
+    P.rates = pj.d_rates;
+    P.x0 = pj.d_x0;
+    P.I = pj.d_I;
+  }
+  if ((e = hipMalloc(&p->d_queue, sizeof(unsigned int) * n_jobs)) != hipSuccess ||
+      (e = hipMalloc(&p->d_stats, sizeof(unsigned long long) * pst::ST_COUNT)) != hipSuccess) {
     pst_plan_destroy(p);
-    return hip_fail(e, "hipMemcpy(tables)");
+    return hip_fail(e, "hipMalloc(queue, stats)");
   }
   if ((e = hipEventCreate(&p->ev0)) != hipSuccess || (e = hipEventCreate(&p->ev1)) != hipSuccess) {
     pst_plan_destroy(p);
     return hip_fail(e, "hipEventCreate");
   }
-  pst::RunArgs& ra = p->ra;
-  std::memset(&ra, 0, sizeof(ra));
-  ra.N = N;
-  ra.M = M;
-  ra.kappa = cfg->kappa;
-  ra.ita = cfg->ita;
-  ra.n_c = cfg->n_c;
-  ra.ssa_burst = cfg->ssa_burst;
-  ra.method = cfg->method;
-  ra.n_sims = cfg->n_sims;
-  ra.sims_per_set = cfg->sims_per_set;
-  ra.eta = cfg->eta;
-  ra.t_max = cfg->t_max;
-  ra.seed = cfg->seed;
-  ra.sim_offset = cfg->sim_offset;
-  ra.rates = p->d_rates;
-  ra.x0 = p->d_x0;
-  ra.I = p->d_I;
-  ra.stats = p->d_stats;
-  ra.precision = cfg->precision;
-  ra.ssa_slice = 4;
   *out = p;
+  return PST_OK;
+}
+
+int pst_plan_create(const pst_network* net, const pst_config* cfg, const double* rates, const int32_t* x0,
+                    int32_t n_sets, pst_plan** out) {
+  g_err[0] = 0;
+  if (!net || !cfg) return fail(PST_EINVAL, "null network or config");
+  pst_job j;
+  j.net = *net;
+  j.cfg = *cfg;
+  j.rates = rates;
+  j.x0 = x0;
+  j.n_sets = n_sets;
+  return pst_batch_plan_create(&j, 1, out);
+}
+
+int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_strides, int32_t* const* flags_dev,
+                           void* stream) {
+  g_err[0] = 0;
+  if (!p) return fail(PST_EINVAL, "null plan");
+  if (!O_dev || !o_strides) return fail(PST_EINVAL, "null trajectory buffer list");
+  int dev = -1;
+  (void)hipGetDevice(&dev);
+  if (dev != p->device) return fail(PST_EINVAL, "plan was created on device %d, current device is %d", p->device, dev);
+  const int n_jobs = (int)p->jobs.size();
+  for (int q = 0; q < n_jobs; ++q) {
+    PlanJob& pj = p->jobs[q];
+    if (pj.p.n_sims > 0 && !O_dev[q]) return fail(PST_EINVAL, "job %d: null trajectory buffer", q);
+    if (o_strides[q] < pj.p.n_sims)
+      return fail(PST_EINVAL, "job %d: o_stride %lld < n_sims %d", q, (long long)o_strides[q], pj.p.n_sims);
+    pj.p.O = O_dev[q];
+    pj.p.o_stride = o_strides[q];
+    pj.p.flags = flags_dev ? flags_dev[q] : nullptr;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  // job descriptors, one device array per size class
+  int count[pst::NUM_CLASSES] = {0, 0, 0};
+  for (int c = 0; c < pst::NUM_CLASSES; ++c) {
+    std::vector<unsigned char> bytes;
+    if (c == 0) pack_jobs<4, 4>(p->jobs, c, bytes, &count[c]);
+    if (c == 1) pack_jobs<8, 8>(p->jobs, c, bytes, &count[c]);
+    if (c == 2) pack_jobs<16, 16>(p->jobs, c, bytes, &count[c]);
+    if (count[c] == 0) continue;
+    if (bytes == p->uploaded[c]) continue;  // unchanged since the last execute
+    if (p->d_jobs_bytes[c] < bytes.size()) {
+      if (p->d_jobs[c]) (void)hipFree(p->d_jobs[c]);
+      p->d_jobs[c] = nullptr;
+      p->d_jobs_bytes[c] = 0;
+      if ((e = hipMalloc(&p->d_jobs[c], bytes.size())) != hipSuccess) return hip_fail(e, "hipMalloc(jobs)");
+      p->d_jobs_bytes[c] = bytes.size();
+    }
+    // the descriptor array may still be read by an earlier launch of this plan
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    if ((e = hipMemcpy(p->d_jobs[c], bytes.data(), bytes.size(), hipMemcpyHostToDevice)) != hipSuccess)
+      return hip_fail(e, "hipMemcpy(jobs)");
+    p->uploaded[c] = bytes;
+  }
+  hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, s, p->d_stats, (int)pst::ST_COUNT);
+  hipLaunchKernelGGL(zero_u32, dim3((n_jobs + 255) / 256), dim3(256), 0, s, p->d_queue, n_jobs);
+  if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "zero kernels");
+  if ((e = hipEventRecord(p->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+  int qoff = 0;
+  for (int c = 0; c < pst::NUM_CLASSES; ++c) {
+    if (count[c] == 0) continue;
+    int64_t need = 0;
+    for (const PlanJob& pj : p->jobs)
+      if (pj.cls == c) need += ((int64_t)pj.p.n_sims + 63) / 64;
+    const int bpc = (p->precision == PST_PRECISION_MIXED) ? pst::blocks_per_cu_mixed(c) : pst::blocks_per_cu_fp64(c);
+    const int64_t resident = (int64_t)bpc * p->cu_count * 4;
+    const int waves = (int)(need < resident ? need : resident);
+    // the queue counters of this class's jobs are queue[qoff .. qoff + count)
+    const int rc = (p->precision == PST_PRECISION_MIXED)
+                       ? pst::launch_mixed(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, s)
+                       : pst::launch_fp64(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, s);
+    if (rc < 0) return fail(PST_ETOOLARGE, "no compiled kernel for size class %d", c);
+    if (rc != 0) return hip_fail((hipError_t)rc, "simulation kernel launch");
+    qoff += count[c];
+  }
+  if ((e = hipEventRecord(p->ev1, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+  p->executed = true;
   return PST_OK;
 }
 
 int pst_plan_execute(pst_plan* p, int32_t* O_dev, int64_t o_stride, int32_t* flags_dev, void* stream) {
   g_err[0] = 0;
   if (!p) return fail(PST_EINVAL, "null plan");
+  if (p->jobs.size() != 1) return fail(PST_EINVAL, "plan holds %d jobs; use pst_batch_plan_execute", (int)p->jobs.size());
   if (!O_dev) return fail(PST_EINVAL, "null trajectory buffer");
-  if (o_stride < p->ra.n_sims) return fail(PST_EINVAL, "o_stride %lld < n_sims %d", (long long)o_stride, p->ra.n_sims);
-  int dev = -1;
-  (void)hipGetDevice(&dev);
-  if (dev != p->device) return fail(PST_EINVAL, "plan was created on device %d, current device is %d", p->device, dev);
-  hipStream_t s = (hipStream_t)stream;
-  pst::RunArgs ra = p->ra;
-  ra.O = O_dev;
-  ra.o_stride = o_stride;
-  ra.flags = flags_dev;
-  hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, s, p->d_stats, (int)pst::ST_COUNT);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return hip_fail(e, "zero_u64 launch");
-  if ((e = hipEventRecord(p->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-  const int rc = (p->ra.precision == PST_PRECISION_MIXED) ? pst::launch_mixed(p->net, ra, s) : pst::launch_fp64(p->net, ra, s);
-  if (rc < 0) return fail(PST_ETOOLARGE, "no compiled kernel for N=%d, M=%d, kappa=%d", p->net.N, p->net.M, p->net.kappa);
-  if (rc != 0) return hip_fail((hipError_t)rc, "simulation kernel launch");
-  if ((e = hipEventRecord(p->ev1, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-  p->executed = true;
-  return PST_OK;
+  int32_t* O[1] = {O_dev};
+  int64_t st[1] = {o_stride};
+  int32_t* F[1] = {flags_dev};
+  return pst_batch_plan_execute(p, O, st, F, stream);
 }
 
 int pst_plan_read_stats(pst_plan* p, pst_stats* st) {
@@ -272,40 +413,69 @@ int pst_plan_read_stats(pst_plan* p, pst_stats* st) {
   return PST_OK;
 }
 
-static int run_host(const pst_network* net, const pst_config* cfg, const double* rates, const int32_t* x0,
-                    int32_t n_sets, int32_t* O, int32_t* flags, pst_stats* stats) {
-  if (!O) return fail(PST_EINVAL, "null trajectory array");
-  pst_plan* p = nullptr;
-  int rc = pst_plan_create(net, cfg, rates, x0, n_sets, &p);
-  if (rc != PST_OK) return rc;
-  const size_t n_out = (size_t)cfg->kappa * cfg->ita * (size_t)cfg->n_sims;
-  int32_t* dO = nullptr;
-  int32_t* dF = nullptr;
-  hipError_t e;
-  if (n_out > 0 && (e = hipMalloc(&dO, n_out * sizeof(int32_t))) != hipSuccess) {
-    pst_plan_destroy(p);
-    return hip_fail(e, "hipMalloc(O)");
+int pst_plan_set_stream(pst_plan* p, uint64_t seed, uint64_t sim_offset) {
+  g_err[0] = 0;
+  if (!p) return fail(PST_EINVAL, "null plan");
+  for (PlanJob& j : p->jobs) {
+    j.p.seed = seed;
+    j.p.sim_offset = sim_offset;
   }
-  if (flags && cfg->n_sims > 0 && (e = hipMalloc(&dF, (size_t)cfg->n_sims * sizeof(int32_t))) != hipSuccess) {
-    (void)hipFree(dO);
-    pst_plan_destroy(p);
-    return hip_fail(e, "hipMalloc(flags)");
+  return PST_OK;
+}
+
+int pst_batch_run(const pst_job* jobs, int32_t n_jobs, int32_t* const* O, int32_t* const* flags, pst_stats* stats) {
+  g_err[0] = 0;
+  if (!O) return fail(PST_EINVAL, "null trajectory array list");
+  pst_plan* p = nullptr;
+  int rc = pst_batch_plan_create(jobs, n_jobs, &p);
+  if (rc != PST_OK) return rc;
+  std::vector<int32_t*> dO(n_jobs, nullptr), dF(n_jobs, nullptr);
+  std::vector<int64_t> strides(n_jobs, 0);
+  hipError_t e = hipSuccess;
+  for (int q = 0; q < n_jobs && rc == PST_OK; ++q) {
+    const pst_config& c = jobs[q].cfg;
+    const size_t n_out = (size_t)c.kappa * c.ita * (size_t)c.n_sims;
+    strides[q] = c.n_sims;
+    if (c.n_sims > 0 && !O[q]) rc = fail(PST_EINVAL, "job %d: null trajectory array", q);
+    else if (n_out > 0 && (e = hipMalloc(&dO[q], n_out * sizeof(int32_t))) != hipSuccess) rc = hip_fail(e, "hipMalloc(O)");
+    else if (flags && flags[q] && c.n_sims > 0 &&
+             (e = hipMalloc(&dF[q], (size_t)c.n_sims * sizeof(int32_t))) != hipSuccess)
+      rc = hip_fail(e, "hipMalloc(flags)");
   }
   pst_stats st = {};
-  if (cfg->n_sims > 0) {
-    rc = pst_plan_execute(p, dO, cfg->n_sims, dF, nullptr);
-    if (rc == PST_OK) rc = pst_plan_read_stats(p, &st);
-    if (rc == PST_OK && (e = hipMemcpy(O, dO, n_out * sizeof(int32_t), hipMemcpyDeviceToHost)) != hipSuccess)
+  if (rc == PST_OK) rc = pst_batch_plan_execute(p, dO.data(), strides.data(), dF.data(), nullptr);
+  if (rc == PST_OK) rc = pst_plan_read_stats(p, &st);
+  for (int q = 0; q < n_jobs && rc == PST_OK; ++q) {
+    const pst_config& c = jobs[q].cfg;
+    const size_t n_out = (size_t)c.kappa * c.ita * (size_t)c.n_sims;
+    if (n_out > 0 && (e = hipMemcpy(O[q], dO[q], n_out * sizeof(int32_t), hipMemcpyDeviceToHost)) != hipSuccess)
       rc = hip_fail(e, "hipMemcpy(O)");
-    if (rc == PST_OK && flags &&
-        (e = hipMemcpy(flags, dF, (size_t)cfg->n_sims * sizeof(int32_t), hipMemcpyDeviceToHost)) != hipSuccess)
+    else if (dF[q] && (e = hipMemcpy(flags[q], dF[q], (size_t)c.n_sims * sizeof(int32_t), hipMemcpyDeviceToHost)) !=
+                          hipSuccess)
       rc = hip_fail(e, "hipMemcpy(flags)");
   }
-  if (dO) (void)hipFree(dO);
-  if (dF) (void)hipFree(dF);
+  for (int q = 0; q < n_jobs; ++q) {
+    if (dO[q]) (void)hipFree(dO[q]);
+    if (dF[q]) (void)hipFree(dF[q]);
+  }
   pst_plan_destroy(p);
   if (rc == PST_OK && stats) *stats = st;
   return rc;
+}
+
+static int run_host(const pst_network* net, const pst_config* cfg, const double* rates, const int32_t* x0,
+                    int32_t n_sets, int32_t* O, int32_t* flags, pst_stats* stats) {
+  if (!net || !cfg) return fail(PST_EINVAL, "null network or config");
+  if (!O) return fail(PST_EINVAL, "null trajectory array");
+  pst_job j;
+  j.net = *net;
+  j.cfg = *cfg;
+  j.rates = rates;
+  j.x0 = x0;
+  j.n_sets = n_sets;
+  int32_t* Os[1] = {O};
+  int32_t* Fs[1] = {flags};
+  return pst_batch_run(&j, 1, Os, Fs, stats);
 }
 
 int pst_tau_leaping_run(const pst_network* net, const pst_config* cfg, const double* rates, const int32_t* x0,

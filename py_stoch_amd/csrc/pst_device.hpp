@@ -58,11 +58,11 @@ __device__ __forceinline__ Words4 philox4x32_10(uint32_t c0, uint32_t c1, uint32
 
 // Per-lane work counters feeding the work model of the roofline figure.
 struct Work {
-  unsigned long long uniforms;   // uniforms drawn
-  unsigned long long ptrs;       // PTRS rejection-loop iterations
-  unsigned long long inv;        // inversion search steps
-  unsigned long long decisions;  // tau-leap decisions (P1_P2 bodies with a0 > 0)
-  unsigned long long samples;    // sample rows written
+  unsigned int uniforms;   // uniforms drawn
+  unsigned int ptrs;       // PTRS rejection-loop iterations
+  unsigned int inv;        // inversion search steps
+  unsigned int decisions;  // tau-leap decisions (P1_P2 bodies with a0 > 0)
+  unsigned int samples;    // sample rows written
 };
 
 // Per-simulation stream key.  Every draw comes from the Philox block

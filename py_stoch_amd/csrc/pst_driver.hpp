@@ -1,0 +1,158 @@
+// pst_driver.hpp -- the persistent, wavefront-uniform simulation driver shared
+// by the FP64 and mixed-precision engines.
+//
+// A launch processes a list of jobs (network + run settings + parameter rows
+// + output array).  Each wavefront works on one job at a time, so the network
+// table is uniform across its lanes; lanes take simulations of that job from
+// a device work queue (one atomic per wavefront per refill) and take the next
+// one as soon as theirs ends, so uneven step counts do not leave lanes idle
+// until the job's queue is empty.  When the job is exhausted and every lane of
+// the wavefront has finished, the wavefront moves to the next job.
+//
+// Per lane the driver keeps a mode -- DECIDE (the P1_P2 body: leap decision
+// and the leap if chosen, P1_P2.py:96-294), SSA (inside a P3 burst,
+// P3.py:71-151) or DONE -- and every pass of its loop runs one DECIDE step for
+// the lanes that need it, then SSA steps for the lanes in a burst: the whole
+// burst if no lane of the wavefront waits to decide, else `ssa_slice` steps.
+// Every simulation executes exactly the reference's sequence (decision, burst
+// of ssa_burst SSA steps, decision, ...); only the interleaving of different
+// simulations within a wavefront changes.
+#pragma once
+#include "pst_device.hpp"
+#include "pst_args.hpp"
+
+namespace pst {
+
+enum Mode { MODE_DECIDE = 0, MODE_SSA = 1, MODE_DONE = 2 };
+
+struct Counters {
+  unsigned long long events;
+  unsigned int leaps, rejected, ssa, flagged;
+  Work w;
+};
+
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0ull; }
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int NP>
+__device__ __forceinline__ int32_t pick_i(const int32_t (&x)[NP], int idx) {
+  int32_t r = x[0];
+#pragma unroll
+  for (int i = 1; i < NP; ++i)
+    if (idx == i) r = x[i];
+  return r;
+}
+
+// Engine interface (see EngineF64 / EngineF32):
+//   int  start(job, sim)          -> initial mode, state loaded
+//   int  decide(job, cnt)         -> next mode (DONE may set flag)
+//   int  ssa_step(job, cnt)       -> 0 continue, 1 ended, 2 overflow
+//   void stop_at_tmax(job, cnt)   -> pure SSA reached t_max
+//   bool before_tmax(job)         -> t < t_max
+//   int  flag
+template <class Engine, int NP, int MP>
+__device__ __forceinline__ void run_driver(const JobDev<NP, MP>* __restrict__ jobs, int n_jobs,
+                                           unsigned int* __restrict__ queue, unsigned long long* __restrict__ stats) {
+  const int lane = (int)(threadIdx.x & 63u);
+  Counters cnt = {};
+  Engine eng;
+  int mode = MODE_DONE;
+  int burst = 0;
+  int64_t sim = -1;
+  int job = 0;
+  bool exhausted = false;
+
+  JobRef<NP, MP> jr;
+  jr.load(jobs);
+
+  for (;;) {
+    // ---- refill lanes whose simulation ended from the job's queue ----
+    const bool want = (mode == MODE_DONE) && !exhausted;
+    const unsigned long long wmask = __builtin_amdgcn_ballot_w64(want);
+    if (wmask != 0ull) {
+      const int n = __builtin_popcountll(wmask);
+      const int leader = __builtin_ctzll(wmask);
+      unsigned int base = 0;
+      if (lane == leader) base = atomicAdd(&queue[job], (unsigned int)n);
+      base = __shfl(base, leader, 64);
+      const int n_sims = jr.p().n_sims;
+      if ((int64_t)base + n >= (int64_t)n_sims) exhausted = true;
+      if (want) {
+        const int rank = __builtin_popcountll(wmask & ((1ull << lane) - 1ull));
+        const int64_t s = (int64_t)base + rank;
+        if (s < n_sims) {
+          sim = s;
+          mode = eng.start(jr, s);
+          burst = 0;
+        }
+      }
+    }
+    if (!wave_any(mode != MODE_DONE)) {
+      // this wavefront is idle and its job has no work left: next job
+      job += 1;
+      if (job >= n_jobs) break;
+      jr.load(jobs + job);
+      exhausted = false;
+      continue;
+    }
+
+    jr.refresh();
+    if (wave_any(mode == MODE_DECIDE)) {
+      if (mode == MODE_DECIDE) {
+        mode = eng.decide(jr, cnt);
+        if (mode == MODE_SSA) burst = jr.p().ssa_burst;
+        if (mode == MODE_DONE) {
+          int32_t* fl = jr.p().flags;
+          if (fl) fl[sim] = eng.flag;
+          cnt.flagged += (eng.flag != 0);
+        }
+      }
+    }
+    const int slice0 = jr.p().ssa_slice;
+    const int slice = slice0 > 0 ? slice0 : 1;
+    const int limit = wave_any(mode == MODE_DECIDE) ? slice : 0x7fffffff;
+    for (int s = 0; s < limit; ++s) {
+      if (!wave_any(mode == MODE_SSA)) break;
+      jr.refresh();
+      if (mode == MODE_SSA) {
+        const int method = jr.p().method;
+        if (method == 2 && !eng.before_tmax(jr)) {
+          // kernel_Gillespie stops at t >= t_max; later samples stay zero (cuGillespie.py:71)
+          eng.stop_at_tmax(jr, cnt);
+          mode = MODE_DONE;
+        } else {
+          const int r = eng.ssa_step(jr, cnt);
+          if (r != 0) {
+            mode = MODE_DONE;
+            if (r == 2) eng.flag = FLAG_OVERFLOW;
+          } else if (method == 0) {
+            burst -= 1;
+            if (burst <= 0) mode = MODE_DECIDE;
+          }
+        }
+        if (mode == MODE_DONE) {
+          int32_t* fl = jr.p().flags;
+          if (fl) fl[sim] = eng.flag;
+          cnt.flagged += (eng.flag != 0);
+        }
+      }
+    }
+  }
+
+  const unsigned long long vals[ST_SAMPLES + 1] = {cnt.events, cnt.leaps, cnt.rejected, cnt.ssa, cnt.flagged,
+                                                   cnt.w.uniforms, cnt.w.ptrs, cnt.w.inv, cnt.w.decisions,
+                                                   cnt.w.samples};
+  // (32-bit per-lane counters: a lane's share of one launch stays far below 2^32)
+#pragma unroll
+  for (int i = 0; i <= ST_SAMPLES; ++i) {
+    const unsigned long long sum = wave_sum(vals[i]);
+    if (lane == 0) atomicAdd(&stats[i], sum);
+  }
+}
+
+}  // namespace pst

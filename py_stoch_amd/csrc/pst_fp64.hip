@@ -19,18 +19,13 @@
 // pst_device.hpp): the first PTRS iteration of every reaction of a leap is
 // evaluated for all reactions together from one group of Philox blocks; only
 // the rare squeeze failures and rejections go through a shared per-lane loop.
-#include "pst_device.hpp"
-#include "pst_args.hpp"
+// The launch loop, work queue and lane modes are in pst_driver.hpp.
+#include "pst_driver.hpp"
 
 namespace pst {
 
-struct Counters {
-  unsigned long long events, leaps, rejected, ssa;
-  Work w;
-};
-
 template <int NP, int MP>
-__device__ __forceinline__ void propensities_f64(const TabView<NP, MP>& tab, const double (&k)[MP],
+__device__ __forceinline__ void propensities_f64(const JobRef<NP, MP>& tab, const double (&k)[MP],
                                                  const int32_t (&x)[NP], double (&a)[MP]) {
   // UpdatePropensities (parser.py:32-55) in table form, species ascending
 #pragma unroll
@@ -132,87 +127,75 @@ __device__ __forceinline__ int64_t inversion_f64(double u, double lam, Work& w) 
     p = p * lam / (double)k;
     F = F + p;
   }
-  w.inv += (unsigned long long)k;
+  w.inv += (unsigned int)k;
   return k;
 }
 
-enum Mode { MODE_DECIDE = 0, MODE_SSA = 1, MODE_DONE = 2 };
-
-__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0ull; }
-
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// One simulation per lane.  The loop structure is wavefront-uniform: every
-// lane is in one of three modes -- DECIDE (the P1_P2 body: leap decision and,
-// if chosen, the leap), SSA (inside a P3 burst of ssa_burst steps) or DONE --
-// and each pass of the outer loop runs one DECIDE step for the lanes that
-// need it and then a slice of SSA steps for the lanes in a burst.  The
-// sequence of operations of every simulation is exactly the reference's
-// (decision, burst of SSA steps, decision, ...); only the interleaving of
-// different simulations within a wavefront differs.
+// Per-lane state and steps of one FP64 simulation.
 template <int NP, int MP>
-__global__ void __launch_bounds__(256) sim_fp64_kernel(const NetTab<NP, MP> ntab, const RunArgs ra) {
-  TabView<NP, MP> tab;
-  tab.load(ntab);
-  const int64_t sim = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = sim < ra.n_sims;
-  const int M = ra.M, N = ra.N;
-  const int kappa = ra.kappa, ita = ra.ita;
-  const int64_t stride = ra.o_stride;
-  const int64_t simc = active ? sim : 0;
-  int32_t* Ob = ra.O + simc;
-  const int64_t set = simc / ra.sims_per_set;
-
-  Counters cnt = {};
-  int flag = 0;
+struct EngineF64 {
   double k[MP];
   int32_t x[NP];
-#pragma unroll
-  for (int j = 0; j < MP; ++j) k[j] = (j < M) ? ra.rates[set * M + j] : 0.0;
-#pragma unroll
-  for (int i = 0; i < NP; ++i) x[i] = (i < N) ? ra.x0[set * N + i] : 0;
-
   Key key;
-  key.init(ra.seed, ra.sim_offset + (uint64_t)simc);
-  uint64_t bc = 0;  // Philox block counter
-  double t = 0.0;
-  int F = 0;
-  double tnext = (double)ra.I[0];
-  int mode = !active ? MODE_DONE : (ra.method == 0 ? MODE_DECIDE : MODE_SSA);
-  int burst = 0;
+  uint64_t bc;  // Philox block counter
+  double t, tnext;
+  int F;
+  int flag;
+  int32_t* Ob;  // &O[0][0][sim]
 
-  auto save = [&](int f, const int32_t (&xs)[NP]) {
-    cnt.w.samples += 1;
+  __device__ __forceinline__ void save(const JobRef<NP, MP>& tab, const JobParams& P, int f,
+                                       const int32_t (&xs)[NP]) const {
 #pragma unroll
     for (int q = 0; q < NP; ++q)
-      if (q < kappa) Ob[((int64_t)q * ita + f) * stride] = pick<NP>(xs, tab.Ev(q));
-  };
-  auto fill_rest = [&](const int32_t (&xs)[NP], bool zero) {
-    cnt.w.samples += (unsigned long long)(ita - F);
+      if (q < P.kappa) Ob[((int64_t)q * P.ita + f) * P.o_stride] = pick_i<NP>(xs, tab.Ev(q));
+  }
+  __device__ __forceinline__ void fill_rest(const JobRef<NP, MP>& tab, const JobParams& P, bool zero,
+                                            Counters& cnt) const {
+    cnt.w.samples += (unsigned int)(P.ita - F);
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
-      if (q < kappa) {
-        const int32_t val = zero ? 0 : pick<NP>(xs, tab.Ev(q));
-        for (int f = F; f < ita; ++f) Ob[((int64_t)q * ita + f) * stride] = val;
+      if (q < P.kappa) {
+        const int32_t val = zero ? 0 : pick_i<NP>(x, tab.Ev(q));
+        for (int f = F; f < P.ita; ++f) Ob[((int64_t)q * P.ita + f) * P.o_stride] = val;
       }
     }
-  };
+  }
 
-  double a[MP];
+  __device__ __forceinline__ int start(const JobRef<NP, MP>& jr, int64_t sim) {
+    const JobParams& P = jr.p();
+    const int64_t set = sim / P.sims_per_set;
+#pragma unroll
+    for (int j = 0; j < MP; ++j) k[j] = (j < P.M) ? P.rates[set * P.M + j] : 0.0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) x[i] = (i < P.N) ? P.x0[set * P.N + i] : 0;
+    key.init(P.seed, P.sim_offset + (uint64_t)sim);
+    bc = 0;
+    t = 0.0;
+    F = 0;
+    tnext = (double)P.I[0];
+    flag = 0;
+    Ob = P.O + sim;
+    return P.method == 0 ? MODE_DECIDE : MODE_SSA;
+  }
 
-  // one direct-method SSA step (P3.py:73-149); returns 1 when the run ended
+  __device__ __forceinline__ bool before_tmax(const JobRef<NP, MP>& jr) const { return t < jr.p().t_max; }
+
+  __device__ __forceinline__ void stop_at_tmax(const JobRef<NP, MP>& tab, Counters& cnt) const {
+    fill_rest(tab, tab.p(), true, cnt);
+  }
+
+  // one direct-method SSA step (P3.py:73-149); 1 when the run ended
   // (absorbed or last sample written), 2 on overflow, 0 otherwise
-  auto ssa_step = [&]() -> int {
+  __device__ __forceinline__ int ssa_step(const JobRef<NP, MP>& tab, Counters& cnt) {
+    const JobParams& P = tab.p();
+    const int M = P.M;
+    double a[MP];
     propensities_f64<NP, MP>(tab, k, x, a);
     double a0 = 0.0;
 #pragma unroll
     for (int j = 0; j < MP; ++j) a0 = a0 + a[j];
     if (a0 == 0.0) {
-      fill_rest(x, false);
+      fill_rest(tab, P, false, cnt);
       return 1;
     }
     const Words4 bw = key.block(bc, 0);
@@ -236,10 +219,11 @@ __global__ void __launch_bounds__(256) sim_fp64_kernel(const NetTab<NP, MP> ntab
     if (sel < 0) sel = last;
     t = t + tau;
     while (t >= tnext) {
-      save(F, x);  // SaveDynamics: pre-reaction state (P3.py:165-177)
+      save(tab, P, F, x);  // SaveDynamics: pre-reaction state (P3.py:165-177)
+      cnt.w.samples += 1;
       F = F + 1;
-      if (F == ita) return 1;
-      tnext = (double)ra.I[F];
+      if (F == P.ita) return 1;
+      tnext = (double)P.I[F];
     }
     // UpdateState (P3.py:196-208)
     bool over = false;
@@ -255,20 +239,23 @@ __global__ void __launch_bounds__(256) sim_fp64_kernel(const NetTab<NP, MP> ntab
     cnt.ssa += 1;
     cnt.events += 1;
     if (over) {
-      fill_rest(x, true);
+      fill_rest(tab, P, true, cnt);
       return 2;
     }
     return 0;
-  };
+  }
 
-  // the P1_P2 body (P1_P2.py:96-294): returns the lane's next mode
-  auto decide = [&]() -> int {
+  // the P1_P2 body (P1_P2.py:96-294): returns the next mode
+  __device__ __forceinline__ int decide(const JobRef<NP, MP>& tab, Counters& cnt) {
+    const JobParams& P = tab.p();
+    const int M = P.M;
+    double a[MP];
     propensities_f64<NP, MP>(tab, k, x, a);
     double a0 = 0.0;
 #pragma unroll
     for (int j = 0; j < MP; ++j) a0 = a0 + a[j];
     if (a0 == 0.0) {
-      fill_rest(x, false);
+      fill_rest(tab, P, false, cnt);
       return MODE_DONE;
     }
     cnt.w.decisions += 1;
@@ -280,7 +267,7 @@ __global__ void __launch_bounds__(256) sim_fp64_kernel(const NetTab<NP, MP> ntab
 #pragma unroll
       for (int i = 0; i < NP; ++i) {
         const int v = tab.V(i, j);
-        if (v < 0) c = c || ((-v) * ra.n_c > x[i]);
+        if (v < 0) c = c || ((-v) * P.n_c > x[i]);
       }
       crit[j] = c;
     }
@@ -303,7 +290,7 @@ __global__ void __launch_bounds__(256) sim_fp64_kernel(const NetTab<NP, MP> ntab
         if (any) {
           // one candidate per V entry of this species; all equal, so one suffices
           const double g_i = calc_g_f64(tab.Hv(i), tab.Htv(i), x[i]);
-          const double num = (double)fmaxf((float)(ra.eta * (double)x[i] / g_i), 1.0f);
+          const double num = (double)fmaxf((float)(P.eta * (double)x[i] / g_i), 1.0f);
           const double lhs = num / (double)fabsf((float)mu);
           const double rhs = (num * num) / s2;
           const double tt = (double)fminf((float)lhs, (float)rhs);
@@ -353,7 +340,7 @@ __global__ void __launch_bounds__(256) sim_fp64_kernel(const NetTab<NP, MP> ntab
         if (sel == j) K[j] = 1;
     }
     if (!(tau < __builtin_inf())) {
-      fill_rest(x, true);
+      fill_rest(tab, P, true, cnt);
       flag = FLAG_TAU_INF;
       return MODE_DONE;
     }
@@ -381,9 +368,9 @@ __global__ void __launch_bounds__(256) sim_fp64_kernel(const NetTab<NP, MP> ntab
           } else {
             cnt.w.uniforms += 2;
             cnt.w.ptrs += 1;
-            const PtrsF64 P = ptrs_setup_f64(lam);
+            const PtrsF64 Pt = ptrs_setup_f64(lam);
             double kf, us;
-            if (ptrs_fast_f64(P, u53(bw.w0, bw.w1) - 0.5, u53(bw.w2, bw.w3), kf, us)) K[j] = (int32_t)kf;
+            if (ptrs_fast_f64(Pt, u53(bw.w0, bw.w1) - 0.5, u53(bw.w2, bw.w3), kf, us)) K[j] = (int32_t)kf;
             else pending |= 1u << j;
           }
         }
@@ -403,7 +390,7 @@ __global__ void __launch_bounds__(256) sim_fp64_kernel(const NetTab<NP, MP> ntab
         }
       }
       if (bad) {
-        fill_rest(x, true);
+        fill_rest(tab, P, true, cnt);
         flag = bad;
         return MODE_DONE;
       }
@@ -428,7 +415,7 @@ __global__ void __launch_bounds__(256) sim_fp64_kernel(const NetTab<NP, MP> ntab
       tau = tau / 2.0;
     }
     if (!valid) {
-      fill_rest(x, true);
+      fill_rest(tab, P, true, cnt);
       flag = FLAG_HALVING;
       return MODE_DONE;
     }
@@ -436,7 +423,7 @@ __global__ void __launch_bounds__(256) sim_fp64_kernel(const NetTab<NP, MP> ntab
 #pragma unroll
     for (int i = 0; i < NP; ++i) big = big || (xp[i] >= COUNT_LIMIT);
     if (big) {
-      fill_rest(x, true);
+      fill_rest(tab, P, true, cnt);
       flag = FLAG_OVERFLOW;
       return MODE_DONE;
     }
@@ -453,93 +440,76 @@ __global__ void __launch_bounds__(256) sim_fp64_kernel(const NetTab<NP, MP> ntab
     bool done = false;
     while (t >= tnext) {
       const double t0 = t - tau;
-      const float record_time = ra.I[F];
+      const float record_time = P.I[F];
       const double gap = t - t0;
       const double nrt = (double)record_time - t0;
       const double frac = nrt / gap;
 #pragma unroll
       for (int q = 0; q < NP; ++q) {
-        if (q < kappa) {
+        if (q < P.kappa) {
           const int s = tab.Ev(q);
-          const int32_t xs = pick<NP>(x, s);
-          const int diff = pick<NP>(xn, s) - xs;
+          const int32_t xs = pick_i<NP>(x, s);
+          const int diff = pick_i<NP>(xn, s) - xs;
           const double inc = (double)diff * frac;
-          Ob[((int64_t)q * ita + F) * stride] = xs + rn_f32_to_int((float)inc);
+          Ob[((int64_t)q * P.ita + F) * P.o_stride] = xs + rn_f32_to_int((float)inc);
         }
       }
       cnt.w.samples += 1;
       F = F + 1;
-      if (F == ita) {
+      if (F == P.ita) {
         done = true;
         break;
       }
-      tnext = (double)ra.I[F];
+      tnext = (double)P.I[F];
     }
 #pragma unroll
     for (int i = 0; i < NP; ++i) x[i] = xn[i];
     return done ? MODE_DONE : MODE_DECIDE;
-  };
-
-  const int slice = ra.ssa_slice > 0 ? ra.ssa_slice : 1;
-  while (wave_any(mode != MODE_DONE)) {
-    tab.refresh();
-    if (wave_any(mode == MODE_DECIDE)) {
-      if (mode == MODE_DECIDE) {
-        mode = decide();
-        if (mode == MODE_SSA) burst = ra.ssa_burst;
-      }
-    }
-    // SSA steps: the whole burst when no lane waits for a decision, else a slice
-    const int limit = wave_any(mode == MODE_DECIDE) ? slice : 0x7fffffff;
-    for (int s = 0; s < limit; ++s) {
-      if (!wave_any(mode == MODE_SSA)) break;
-      tab.refresh();
-      if (mode == MODE_SSA) {
-        if (ra.method == 2 && !(t < ra.t_max)) {
-          // kernel_Gillespie stops at t >= t_max; later samples stay zero (cuGillespie.py:71)
-          fill_rest(x, true);
-          mode = MODE_DONE;
-        } else {
-          const int r = ssa_step();
-          if (r != 0) {
-            mode = MODE_DONE;
-            if (r == 2) flag = FLAG_OVERFLOW;
-          } else if (ra.method == 0) {
-            burst -= 1;
-            if (burst == 0) mode = MODE_DECIDE;
-          }
-        }
-      }
-    }
   }
+};
 
-  if (active && ra.flags) ra.flags[sim] = flag;
-  const unsigned long long flagged = (active && flag != 0) ? 1ull : 0ull;
-  const unsigned long long vals[ST_SAMPLES + 1] = {cnt.events, cnt.leaps, cnt.rejected, cnt.ssa, flagged,
-                                                   cnt.w.uniforms, cnt.w.ptrs, cnt.w.inv, cnt.w.decisions,
-                                                   cnt.w.samples};
-#pragma unroll
-  for (int i = 0; i <= ST_SAMPLES; ++i) {
-    const unsigned long long sum = wave_sum(vals[i]);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&ra.stats[i], sum);
-  }
+template <int NP, int MP>
+__global__ void __launch_bounds__(256) sim_fp64_kernel(const JobDev<NP, MP>* __restrict__ jobs, int n_jobs,
+                                                       unsigned int* __restrict__ queue,
+                                                       unsigned long long* __restrict__ stats) {
+  run_driver<EngineF64<NP, MP>, NP, MP>(jobs, n_jobs, queue, stats);
 }
 
 template <int NP, int MP>
-static int launch_class(const HostNet& net, const RunArgs& ra, hipStream_t stream) {
-  const NetTab<NP, MP> tab = make_tab<NP, MP>(net);
+static int launch_cls(const void* jobs, int n_jobs, unsigned int* queue, unsigned long long* stats, int waves,
+                      hipStream_t stream) {
   const int block = 256;
-  const int grid = (int)((ra.n_sims + block - 1) / block);
-  if (grid == 0) return 0;
-  hipLaunchKernelGGL((sim_fp64_kernel<NP, MP>), dim3(grid), dim3(block), 0, stream, tab, ra);
+  const int grid = (waves + 3) / 4;
+  if (grid <= 0) return 0;
+  hipLaunchKernelGGL((sim_fp64_kernel<NP, MP>), dim3(grid), dim3(block), 0, stream,
+                     (const JobDev<NP, MP>*)jobs, n_jobs, queue, stats);
   return (int)hipGetLastError();
 }
 
-int launch_fp64(const HostNet& net, const RunArgs& ra, hipStream_t stream) {
-  if (net.N <= 4 && net.M <= 4 && net.kappa <= 4) return launch_class<4, 4>(net, ra, stream);
-  if (net.N <= 8 && net.M <= 8 && net.kappa <= 8) return launch_class<8, 8>(net, ra, stream);
-  if (net.N <= 16 && net.M <= 16 && net.kappa <= 16) return launch_class<16, 16>(net, ra, stream);
-  return -1;
+template <int NP, int MP>
+static int occupancy_cls() {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, sim_fp64_kernel<NP, MP>, 256, 0) != hipSuccess) return 1;
+  return blocks > 0 ? blocks : 1;
+}
+
+int launch_fp64(int cls, const void* jobs, int n_jobs, unsigned int* queue, unsigned long long* stats, int waves,
+                hipStream_t stream) {
+  switch (cls) {
+    case 0: return launch_cls<4, 4>(jobs, n_jobs, queue, stats, waves, stream);
+    case 1: return launch_cls<8, 8>(jobs, n_jobs, queue, stats, waves, stream);
+    case 2: return launch_cls<16, 16>(jobs, n_jobs, queue, stats, waves, stream);
+    default: return -1;
+  }
+}
+
+int blocks_per_cu_fp64(int cls) {
+  switch (cls) {
+    case 0: return occupancy_cls<4, 4>();
+    case 1: return occupancy_cls<8, 8>();
+    case 2: return occupancy_cls<16, 16>();
+    default: return 1;
+  }
 }
 
 }  // namespace pst

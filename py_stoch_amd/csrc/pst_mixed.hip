@@ -12,19 +12,14 @@
 //         with sum_{r<=j} a_r >= rho2 * a0" accumulated in FP64 so that FP32
 //         round-off cannot bias which reaction fires.  Poisson means of 2^22
 //         and above use the FP64 sampler (FP32 cannot resolve the integers).
-// Transcendentals use the hardware v_log_f32 / v_exp_f32 / v_rcp_f32.
-#include "pst_device.hpp"
-#include "pst_args.hpp"
+// Transcendentals use the hardware v_log_f32 / v_exp_f32 / v_rcp_f32.  The
+// launch loop, work queue and lane modes are in pst_driver.hpp.
+#include "pst_driver.hpp"
 
 namespace pst {
 
-struct CountersM {
-  unsigned long long events, leaps, rejected, ssa;
-  Work w;
-};
-
 template <int NP, int MP>
-__device__ __forceinline__ void propensities_f32(const TabView<NP, MP>& tab, const float (&k)[MP],
+__device__ __forceinline__ void propensities_f32(const JobRef<NP, MP>& tab, const float (&k)[MP],
                                                  const int32_t (&x)[NP], float (&a)[MP]) {
 #pragma unroll
   for (int j = 0; j < MP; ++j) {
@@ -116,7 +111,7 @@ __device__ __noinline__ bool ptrs_slow_f32(const PtrsF32& P, float V, float kf, 
 
 // Full PTRS draw from block c, words of half h (iterations it = 0, 1, ...)
 __device__ __noinline__ int64_t ptrs_full_f32(uint32_t k0, uint32_t k1, uint32_t s_lo, uint32_t s_hi, uint64_t c,
-                                              int h, float lam, unsigned long long* wptrs) {
+                                              int h, float lam, unsigned int* wptrs) {
   Key key;
   key.k0 = k0;
   key.k1 = k1;
@@ -137,7 +132,7 @@ __device__ __noinline__ int64_t ptrs_full_f32(uint32_t k0, uint32_t k1, uint32_t
 
 // Poisson means too large for FP32 integer resolution: PTRS in FP64
 __device__ __noinline__ int64_t ptrs_big_f64(uint32_t k0, uint32_t k1, uint32_t s_lo, uint32_t s_hi, uint64_t c,
-                                             int h, double lam, unsigned long long* wptrs) {
+                                             int h, double lam, unsigned int* wptrs) {
   Key key;
   key.k0 = k0;
   key.k1 = k1;
@@ -173,81 +168,75 @@ __device__ __forceinline__ int64_t inversion_f32(float u, float lam, Work& w) {
     p = p * lam * fast_rcp((float)k);
     F = F + p;
   }
-  w.inv += (unsigned long long)k;
+  w.inv += (unsigned int)k;
   return k;
-}
-
-enum ModeM { MMODE_DECIDE = 0, MMODE_SSA = 1, MMODE_DONE = 2 };
-
-__device__ __forceinline__ bool wave_any_m(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0ull; }
-
-__device__ __forceinline__ unsigned long long wave_sum_m(unsigned long long v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
 }
 
 constexpr float BIG_LAMBDA_F32 = 4194304.0f;  // 2^22
 
+// Per-lane state and steps of one mixed-precision simulation.
 template <int NP, int MP>
-__global__ void __launch_bounds__(256) sim_mixed_kernel(const NetTab<NP, MP> ntab, const RunArgs ra) {
-  TabView<NP, MP> tab;
-  tab.load(ntab);
-  const int64_t sim = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = sim < ra.n_sims;
-  const int M = ra.M, N = ra.N;
-  const int kappa = ra.kappa, ita = ra.ita;
-  const int64_t stride = ra.o_stride;
-  const int64_t simc = active ? sim : 0;
-  int32_t* Ob = ra.O + simc;
-  const int64_t set = simc / ra.sims_per_set;
-  const float eta = (float)ra.eta;
-
-  CountersM cnt = {};
-  int flag = 0;
+struct EngineF32 {
   float k[MP];
   int32_t x[NP];
-#pragma unroll
-  for (int j = 0; j < MP; ++j) k[j] = (j < M) ? (float)ra.rates[set * M + j] : 0.0f;
-#pragma unroll
-  for (int i = 0; i < NP; ++i) x[i] = (i < N) ? ra.x0[set * N + i] : 0;
-
   Key key;
-  key.init(ra.seed, ra.sim_offset + (uint64_t)simc);
-  uint64_t bc = 0;
-  double t = 0.0;
-  int F = 0;
-  double tnext = (double)ra.I[0];
-  int mode = !active ? MMODE_DONE : (ra.method == 0 ? MMODE_DECIDE : MMODE_SSA);
-  int burst = 0;
-  const int nblk = (M + 1) >> 1;  // Philox blocks per Poisson attempt
+  uint64_t bc;
+  double t, tnext;
+  int F;
+  int flag;
+  int32_t* Ob;
 
-  auto save = [&](int f, const int32_t (&xs)[NP]) {
-    cnt.w.samples += 1;
+  __device__ __forceinline__ void save(const JobRef<NP, MP>& tab, const JobParams& P, int f,
+                                       const int32_t (&xs)[NP]) const {
 #pragma unroll
     for (int q = 0; q < NP; ++q)
-      if (q < kappa) Ob[((int64_t)q * ita + f) * stride] = pick_m<NP>(xs, tab.Ev(q));
-  };
-  auto fill_rest = [&](const int32_t (&xs)[NP], bool zero) {
-    cnt.w.samples += (unsigned long long)(ita - F);
+      if (q < P.kappa) Ob[((int64_t)q * P.ita + f) * P.o_stride] = pick_i<NP>(xs, tab.Ev(q));
+  }
+  __device__ __forceinline__ void fill_rest(const JobRef<NP, MP>& tab, const JobParams& P, bool zero,
+                                            Counters& cnt) const {
+    cnt.w.samples += (unsigned int)(P.ita - F);
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
-      if (q < kappa) {
-        const int32_t val = zero ? 0 : pick_m<NP>(xs, tab.Ev(q));
-        for (int f = F; f < ita; ++f) Ob[((int64_t)q * ita + f) * stride] = val;
+      if (q < P.kappa) {
+        const int32_t val = zero ? 0 : pick_i<NP>(x, tab.Ev(q));
+        for (int f = F; f < P.ita; ++f) Ob[((int64_t)q * P.ita + f) * P.o_stride] = val;
       }
     }
-  };
+  }
 
-  float a[MP];
+  __device__ __forceinline__ int start(const JobRef<NP, MP>& jr, int64_t sim) {
+    const JobParams& P = jr.p();
+    const int64_t set = sim / P.sims_per_set;
+#pragma unroll
+    for (int j = 0; j < MP; ++j) k[j] = (j < P.M) ? (float)P.rates[set * P.M + j] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) x[i] = (i < P.N) ? P.x0[set * P.N + i] : 0;
+    key.init(P.seed, P.sim_offset + (uint64_t)sim);
+    bc = 0;
+    t = 0.0;
+    F = 0;
+    tnext = (double)P.I[0];
+    flag = 0;
+    Ob = P.O + sim;
+    return P.method == 0 ? MODE_DECIDE : MODE_SSA;
+  }
 
-  auto ssa_step = [&]() -> int {
+  __device__ __forceinline__ bool before_tmax(const JobRef<NP, MP>& jr) const { return t < jr.p().t_max; }
+
+  __device__ __forceinline__ void stop_at_tmax(const JobRef<NP, MP>& tab, Counters& cnt) const {
+    fill_rest(tab, tab.p(), true, cnt);
+  }
+
+  __device__ __forceinline__ int ssa_step(const JobRef<NP, MP>& tab, Counters& cnt) {
+    const JobParams& P = tab.p();
+    const int M = P.M;
+    float a[MP];
     propensities_f32<NP, MP>(tab, k, x, a);
     double a0d = 0.0;
 #pragma unroll
     for (int j = 0; j < MP; ++j) a0d += (double)a[j];
     if (a0d == 0.0) {
-      fill_rest(x, false);
+      fill_rest(tab, P, false, cnt);
       return 1;
     }
     const Words4 bw = key.block(bc, 0);
@@ -268,10 +257,11 @@ __global__ void __launch_bounds__(256) sim_mixed_kernel(const NetTab<NP, MP> nta
     if (sel < 0) sel = last;
     t = t + (double)tau;
     while (t >= tnext) {
-      save(F, x);
+      save(tab, P, F, x);
+      cnt.w.samples += 1;
       F = F + 1;
-      if (F == ita) return 1;
-      tnext = (double)ra.I[F];
+      if (F == P.ita) return 1;
+      tnext = (double)P.I[F];
     }
     bool over = false;
 #pragma unroll
@@ -286,22 +276,28 @@ __global__ void __launch_bounds__(256) sim_mixed_kernel(const NetTab<NP, MP> nta
     cnt.ssa += 1;
     cnt.events += 1;
     if (over) {
-      fill_rest(x, true);
+      fill_rest(tab, P, true, cnt);
       return 2;
     }
     return 0;
-  };
+  }
 
-  auto decide = [&]() -> int {
+  __device__ __forceinline__ int decide(const JobRef<NP, MP>& tab, Counters& cnt) {
+    const JobParams& P = tab.p();
+    const int M = P.M;
+    const float eta = (float)P.eta;
+    float a[MP];
     propensities_f32<NP, MP>(tab, k, x, a);
     float a0 = 0.0f;
 #pragma unroll
     for (int j = 0; j < MP; ++j) a0 += a[j];
     if (a0 == 0.0f) {
-      fill_rest(x, false);
-      return MMODE_DONE;
+      fill_rest(tab, P, false, cnt);
+      return MODE_DONE;
     }
     cnt.w.decisions += 1;
+    // critical reactions (P1_P2.py:314-327): per-lane conditions as selects,
+    // branches only on table entries (uniform)
     bool crit[MP];
 #pragma unroll
     for (int j = 0; j < MP; ++j) {
@@ -309,7 +305,7 @@ __global__ void __launch_bounds__(256) sim_mixed_kernel(const NetTab<NP, MP> nta
 #pragma unroll
       for (int i = 0; i < NP; ++i) {
         const int v = tab.V(i, j);
-        if (v < 0) c = c || ((-v) * ra.n_c > x[i]);
+        if (v < 0) c = c | ((-v) * P.n_c > x[i]);
       }
       crit[j] = c;
     }
@@ -322,30 +318,28 @@ __global__ void __launch_bounds__(256) sim_mixed_kernel(const NetTab<NP, MP> nta
 #pragma unroll
         for (int j = 0; j < MP; ++j) {
           const int v = tab.V(i, j);
-          if (v != 0 && !crit[j]) {
-            mu += (float)v * a[j];
-            s2 += (float)(v * v) * a[j];
-            any = true;
+          if (v != 0) {
+            const bool nc = !crit[j];
+            mu += nc ? (float)v * a[j] : 0.0f;
+            s2 += nc ? (float)(v * v) * a[j] : 0.0f;
+            any = any | nc;
           }
         }
-        if (any) {
-          const float g_i = calc_g_f32(tab.Hv(i), tab.Htv(i), x[i]);
-          const float num = fmaxf(eta * (float)x[i] * fast_rcp(g_i), 1.0f);
-          const float lhs = num * fast_rcp(fabsf(mu));
-          const float rhs = num * num * fast_rcp(s2);
-          tau1 = fminf(tau1, fminf(lhs, rhs));
-        }
+        const float g_i = calc_g_f32(tab.Hv(i), tab.Htv(i), x[i]);
+        const float num = fmaxf(eta * (float)x[i] * fast_rcp(g_i), 1.0f);
+        const float cand = fminf(num * fast_rcp(fabsf(mu)), num * num * fast_rcp(s2));
+        tau1 = any ? fminf(tau1, cand) : tau1;
       }
     }
-    if (tau1 < 10.0f * fast_rcp(a0)) return MMODE_SSA;
+    if (tau1 < 10.0f * fast_rcp(a0)) return MODE_SSA;
 
     int32_t K[MP];
-#pragma unroll
-    for (int j = 0; j < MP; ++j) K[j] = 0;
     float a0c = 0.0f;
 #pragma unroll
-    for (int j = 0; j < MP; ++j)
-      if (crit[j]) a0c += a[j];
+    for (int j = 0; j < MP; ++j) {
+      K[j] = 0;
+      a0c += crit[j] ? a[j] : 0.0f;
+    }
     float tau2 = __builtin_inff();
     uint32_t cw1 = 0;
     if (a0c > 0.0f) {
@@ -371,90 +365,129 @@ __global__ void __launch_bounds__(256) sim_mixed_kernel(const NetTab<NP, MP> nta
       }
       if (sel < 0) sel = last;
 #pragma unroll
-      for (int j = 0; j < MP; ++j)
-        if (sel == j) K[j] = 1;
+      for (int j = 0; j < MP; ++j) K[j] = (sel == j) ? 1 : K[j];
     }
     if (!(tau < __builtin_inff())) {
-      fill_rest(x, true);
+      fill_rest(tab, P, true, cnt);
       flag = FLAG_TAU_INF;
-      return MMODE_DONE;
+      return MODE_DONE;
     }
+    constexpr int NB = (MP + 1) / 2;  // Philox blocks per Poisson attempt (two reactions per block)
+    const int nblk = (M + 1) >> 1;
     int64_t xp[NP];
     bool valid = false;
     for (int h = 0; h < MAX_HALVINGS; ++h) {
       const uint64_t grp = bc;
       bc += (uint64_t)nblk;
-      int bad = 0;
-      unsigned pending = 0;
-      // first draw of every non-critical reaction: reactions 2r and 2r+1 share a block
+      // every reaction's first draw, straight-line: independent Philox blocks and
+      // PTRS squeezes that the scheduler interleaves
+      Words4 W[NB];
 #pragma unroll
-      for (int r = 0; r < (MP + 1) / 2; ++r) {
-        if (2 * r < M) {
-          const Words4 bw = key.block(grp + (uint64_t)r, 0);
+      for (int r = 0; r < NB; ++r) W[r] = key.block(grp + (uint64_t)r, 0);
+      float lam[MP];
+      unsigned pend_ptrs = 0, pend_inv = 0, drawn = 0;
+      bool bad = false;
 #pragma unroll
-          for (int hh = 0; hh < 2; ++hh) {
-            const int j = 2 * r + hh;
-            if (j < MP && j < M && !crit[j]) {
-              const float lam = a[j] * tau;
-              const uint32_t wu = hh ? bw.w2 : bw.w0;
-              const uint32_t wv = hh ? bw.w3 : bw.w1;
-              if (!(lam > 0.0f)) {
-                K[j] = 0;
-              } else if (lam >= (float)LAMBDA_LIMIT) {
-                bad = FLAG_OVERFLOW;
-              } else if (lam < 10.0f) {
-                cnt.w.uniforms += 1;
-                K[j] = (int32_t)inversion_f32(u24(wu), lam, cnt.w);
-              } else if (lam >= BIG_LAMBDA_F32) {
-                pending |= 1u << j;
-              } else {
-                cnt.w.uniforms += 2;
-                cnt.w.ptrs += 1;
-                const PtrsF32 P = ptrs_setup_f32(lam);
-                float kf, us;
-                if (ptrs_fast_f32(P, u24(wu) - 0.5f, u24(wv), kf, us)) K[j] = (int32_t)kf;
-                else pending |= 1u << j;
-              }
-            }
+      for (int j = 0; j < MP; ++j) {
+        const bool live = (j < M) && !crit[j];
+        const float l = live ? a[j] * tau : 0.0f;
+        lam[j] = l;
+        const uint32_t wu = (j & 1) ? W[j >> 1].w2 : W[j >> 1].w0;
+        const uint32_t wv = (j & 1) ? W[j >> 1].w3 : W[j >> 1].w1;
+        const PtrsF32 Pt = ptrs_setup_f32(fmaxf(l, 10.0f));
+        float kf, us;
+        const bool fast = ptrs_fast_f32(Pt, u24(wu) - 0.5f, u24(wv), kf, us);
+        const bool use_ptrs = (l >= 10.0f) & (l < BIG_LAMBDA_F32);
+        const bool ok = use_ptrs & fast;
+        if (!crit[j]) K[j] = ok ? (int32_t)kf : 0;
+        pend_ptrs |= ((l >= 10.0f) & !ok) ? (1u << j) : 0u;
+        drawn |= use_ptrs ? (1u << j) : 0u;
+        pend_inv |= ((l > 0.0f) & (l < 10.0f)) ? (1u << j) : 0u;
+        bad = bad | (l >= (float)LAMBDA_LIMIT);
+      }
+      cnt.w.ptrs += (unsigned int)__builtin_popcount(drawn);
+      cnt.w.uniforms += 2u * (unsigned int)__builtin_popcount(drawn);
+      if (bad) {
+        fill_rest(tab, P, true, cnt);
+        flag = FLAG_OVERFLOW;
+        return MODE_DONE;
+      }
+      // small means: sequential-search inversion for all such reactions together
+      if (pend_inv != 0u) {
+        float pj[MP], Fj[MP], uj[MP];
+        int kj[MP];
+        unsigned act = 0;
+#pragma unroll
+        for (int j = 0; j < MP; ++j) {
+          const bool on = (pend_inv >> j) & 1u;
+          uj[j] = u24((j & 1) ? W[j >> 1].w2 : W[j >> 1].w0);
+          pj[j] = fast_exp(-lam[j]);
+          Fj[j] = pj[j];
+          kj[j] = 0;
+          act |= (on && uj[j] > Fj[j]) ? (1u << j) : 0u;
+        }
+        cnt.w.uniforms += (unsigned int)__builtin_popcount(pend_inv);
+        int it = 0;
+        while (act != 0u && it < INV_MAX_K) {
+          it += 1;
+#pragma unroll
+          for (int j = 0; j < MP; ++j) {
+            const bool on = (act >> j) & 1u;
+            const int kn = kj[j] + 1;
+            const float pn = pj[j] * lam[j] * fast_rcp((float)kn);
+            const float Fn = Fj[j] + pn;
+            kj[j] = on ? kn : kj[j];
+            pj[j] = on ? pn : pj[j];
+            Fj[j] = on ? Fn : Fj[j];
+            act = (on && !(uj[j] > Fn)) ? (act & ~(1u << j)) : act;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < MP; ++j) {
+          if ((pend_inv >> j) & 1u) {
+            K[j] = kj[j];
+            cnt.w.inv += (unsigned int)kj[j];
           }
         }
       }
-      while (pending != 0u && !bad) {
-        const int j = __builtin_ctz(pending);
-        pending &= pending - 1u;
-        const float lam = pick_m<MP>(a, j) * tau;
-        unsigned long long extra = 0;
-        const int64_t kk = (lam >= BIG_LAMBDA_F32)
+      // squeeze failures and very large means: the full PTRS draw, one shared loop
+      bool ptrs_fail = false;
+      while (pend_ptrs != 0u && !ptrs_fail) {
+        const int j = __builtin_ctz(pend_ptrs);
+        pend_ptrs &= pend_ptrs - 1u;
+        const float l = pick_m<MP>(lam, j);
+        unsigned int extra = 0;
+        const int64_t kk = (l >= BIG_LAMBDA_F32)
                                ? ptrs_big_f64(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1,
-                                              (double)lam, &extra)
+                                              (double)l, &extra)
                                : ptrs_full_f32(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1,
-                                               lam, &extra);
+                                               l, &extra);
         cnt.w.ptrs += extra;
         cnt.w.uniforms += 2 * extra;
         if (kk < 0) {
-          bad = FLAG_PTRS;
+          ptrs_fail = true;
         } else {
 #pragma unroll
-          for (int r = 0; r < MP; ++r)
-            if (r == j) K[r] = (int32_t)kk;
+          for (int r = 0; r < MP; ++r) K[r] = (r == j) ? (int32_t)kk : K[r];
         }
       }
-      if (bad) {
-        fill_rest(x, true);
-        flag = bad;
-        return MMODE_DONE;
+      if (ptrs_fail) {
+        fill_rest(tab, P, true, cnt);
+        flag = FLAG_PTRS;
+        return MODE_DONE;
       }
+      // TentativeUpdatedState / ValidState (P1_P2.py:509-540)
       bool ok = true;
 #pragma unroll
       for (int i = 0; i < NP; ++i) {
-        int64_t s = x[i];
+        int64_t sacc = x[i];
 #pragma unroll
         for (int j = 0; j < MP; ++j) {
           const int v = tab.V(i, j);
-          if (v != 0) s += (int64_t)K[j] * v;
+          if (v != 0) sacc += (int64_t)K[j] * v;
         }
-        xp[i] = s;
-        ok = ok && (s >= 0);
+        xp[i] = sacc;
+        ok = ok & (sacc >= 0);
       }
       if (ok) {
         valid = true;
@@ -464,17 +497,17 @@ __global__ void __launch_bounds__(256) sim_mixed_kernel(const NetTab<NP, MP> nta
       tau = tau * 0.5f;
     }
     if (!valid) {
-      fill_rest(x, true);
+      fill_rest(tab, P, true, cnt);
       flag = FLAG_HALVING;
-      return MMODE_DONE;
+      return MODE_DONE;
     }
     bool big = false;
 #pragma unroll
-    for (int i = 0; i < NP; ++i) big = big || (xp[i] >= COUNT_LIMIT);
+    for (int i = 0; i < NP; ++i) big = big | (xp[i] >= COUNT_LIMIT);
     if (big) {
-      fill_rest(x, true);
+      fill_rest(tab, P, true, cnt);
       flag = FLAG_OVERFLOW;
-      return MMODE_DONE;
+      return MODE_DONE;
     }
     unsigned long long ev = 0;
 #pragma unroll
@@ -493,85 +526,70 @@ __global__ void __launch_bounds__(256) sim_mixed_kernel(const NetTab<NP, MP> nta
         const float frac = (float)(tnext - t0) * inv_gap;
 #pragma unroll
         for (int q = 0; q < NP; ++q) {
-          if (q < kappa) {
+          if (q < P.kappa) {
             const int s = tab.Ev(q);
-            const int32_t xs = pick_m<NP>(x, s);
-            const int diff = pick_m<NP>(xn, s) - xs;
-            Ob[((int64_t)q * ita + F) * stride] = xs + rn_f32_to_int((float)diff * frac);
+            const int32_t xs = pick_i<NP>(x, s);
+            const int diff = pick_i<NP>(xn, s) - xs;
+            Ob[((int64_t)q * P.ita + F) * P.o_stride] = xs + rn_f32_to_int((float)diff * frac);
           }
         }
         cnt.w.samples += 1;
         F = F + 1;
-        if (F == ita) {
+        if (F == P.ita) {
           done = true;
           break;
         }
-        tnext = (double)ra.I[F];
+        tnext = (double)P.I[F];
       }
     }
 #pragma unroll
     for (int i = 0; i < NP; ++i) x[i] = xn[i];
-    return done ? MMODE_DONE : MMODE_DECIDE;
-  };
-
-  const int slice = ra.ssa_slice > 0 ? ra.ssa_slice : 1;
-  while (wave_any_m(mode != MMODE_DONE)) {
-    tab.refresh();
-    if (wave_any_m(mode == MMODE_DECIDE)) {
-      if (mode == MMODE_DECIDE) {
-        mode = decide();
-        if (mode == MMODE_SSA) burst = ra.ssa_burst;
-      }
-    }
-    const int limit = wave_any_m(mode == MMODE_DECIDE) ? slice : 0x7fffffff;
-    for (int s = 0; s < limit; ++s) {
-      if (!wave_any_m(mode == MMODE_SSA)) break;
-      tab.refresh();
-      if (mode == MMODE_SSA) {
-        if (ra.method == 2 && !(t < ra.t_max)) {
-          fill_rest(x, true);
-          mode = MMODE_DONE;
-        } else {
-          const int r = ssa_step();
-          if (r != 0) {
-            mode = MMODE_DONE;
-            if (r == 2) flag = FLAG_OVERFLOW;
-          } else if (ra.method == 0) {
-            burst -= 1;
-            if (burst == 0) mode = MMODE_DECIDE;
-          }
-        }
-      }
-    }
+    return done ? MODE_DONE : MODE_DECIDE;
   }
+};
 
-  if (active && ra.flags) ra.flags[sim] = flag;
-  const unsigned long long flagged = (active && flag != 0) ? 1ull : 0ull;
-  const unsigned long long vals[ST_SAMPLES + 1] = {cnt.events, cnt.leaps, cnt.rejected, cnt.ssa, flagged,
-                                                   cnt.w.uniforms, cnt.w.ptrs, cnt.w.inv, cnt.w.decisions,
-                                                   cnt.w.samples};
-#pragma unroll
-  for (int i = 0; i <= ST_SAMPLES; ++i) {
-    const unsigned long long sum = wave_sum_m(vals[i]);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&ra.stats[i], sum);
-  }
+template <int NP, int MP>
+__global__ void __launch_bounds__(256) sim_mixed_kernel(const JobDev<NP, MP>* __restrict__ jobs, int n_jobs,
+                                                        unsigned int* __restrict__ queue,
+                                                        unsigned long long* __restrict__ stats) {
+  run_driver<EngineF32<NP, MP>, NP, MP>(jobs, n_jobs, queue, stats);
 }
 
 template <int NP, int MP>
-static int launch_class_m(const HostNet& net, const RunArgs& ra, hipStream_t stream) {
-  const NetTab<NP, MP> tab = make_tab<NP, MP>(net);
+static int launch_cls_m(const void* jobs, int n_jobs, unsigned int* queue, unsigned long long* stats, int waves,
+                        hipStream_t stream) {
   const int block = 256;
-  const int grid = (int)((ra.n_sims + block - 1) / block);
-  if (grid == 0) return 0;
-  hipLaunchKernelGGL((sim_mixed_kernel<NP, MP>), dim3(grid), dim3(block), 0, stream, tab, ra);
+  const int grid = (waves + 3) / 4;
+  if (grid <= 0) return 0;
+  hipLaunchKernelGGL((sim_mixed_kernel<NP, MP>), dim3(grid), dim3(block), 0, stream,
+                     (const JobDev<NP, MP>*)jobs, n_jobs, queue, stats);
   return (int)hipGetLastError();
 }
 
-int launch_mixed(const HostNet& net, const RunArgs& ra, hipStream_t stream) {
-  if (net.N <= 4 && net.M <= 4 && net.kappa <= 4) return launch_class_m<4, 4>(net, ra, stream);
-  if (net.N <= 8 && net.M <= 8 && net.kappa <= 8) return launch_class_m<8, 8>(net, ra, stream);
-  if (net.N <= 16 && net.M <= 16 && net.kappa <= 16) return launch_class_m<16, 16>(net, ra, stream);
-  return -1;
+template <int NP, int MP>
+static int occupancy_cls_m() {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, sim_mixed_kernel<NP, MP>, 256, 0) != hipSuccess) return 1;
+  return blocks > 0 ? blocks : 1;
+}
+
+int launch_mixed(int cls, const void* jobs, int n_jobs, unsigned int* queue, unsigned long long* stats, int waves,
+                 hipStream_t stream) {
+  switch (cls) {
+    case 0: return launch_cls_m<4, 4>(jobs, n_jobs, queue, stats, waves, stream);
+    case 1: return launch_cls_m<8, 8>(jobs, n_jobs, queue, stats, waves, stream);
+    case 2: return launch_cls_m<16, 16>(jobs, n_jobs, queue, stats, waves, stream);
+    default: return -1;
+  }
+}
+
+int blocks_per_cu_mixed(int cls) {
+  switch (cls) {
+    case 0: return occupancy_cls_m<4, 4>();
+    case 1: return occupancy_cls_m<8, 8>();
+    case 2: return occupancy_cls_m<16, 16>();
+    default: return 1;
+  }
 }
 
 }  // namespace pst

@@ -73,3 +73,42 @@ def test_sim_offset_reproducible(native, oracle_mod):
     full, _, _ = native.simulate(tabs, rates, a.x_0, n_sims=128, **kw)
     tail, _, _ = native.simulate(tabs, rates, a.x_0, n_sims=64, sim_offset=64, **kw)
     assert np.array_equal(full[:, :, 64:], tail)
+
+
+def test_batch_equals_separate_runs(native, oracle_mod):
+    """Jobs of different networks and parameter rows in one persistent launch
+    give, job by job, the same bits as the CPU oracle run of that job."""
+    rng = np.random.default_rng(7)
+    specs = []
+    a_lv = models.make_args(models.lotka_volterra(), E=[0, 1], t_max=2.0, ita=21, U=64)
+    rates_lv = np.array([a_lv.rate_fn(a_lv.c) * 10.0 ** rng.uniform(-0.3, 0.3, 3) for _ in range(6)])
+    specs.append((a_lv, rates_lv, np.tile(a_lv.x_0, (6, 1)), 6 * 100, 100, 0))
+    a_sc = models.make_args(models.schlogl(), E=[0], t_max=2.0, ita=11, U=64)
+    rates_sc = np.array([a_sc.rate_fn(a_sc.c) * 10.0 ** rng.uniform(-0.2, 0.2, 4) for _ in range(4)])
+    specs.append((a_sc, rates_sc, np.tile(a_sc.x_0, (4, 1)), 4 * 90, 90, 0))
+    a_bd = models.make_args(models.birth_death(), E=[0], t_max=20.0, ita=21, U=64)
+    specs.append((a_bd, a_bd.rate_fn(a_bd.c)[None, :], a_bd.x_0[None, :], 333, 333, 2))
+    jobs, refs = [], []
+    for a, rates, x0, n, per, method in specs:
+        kw = dict(method=method, t_max=float(a.t_max), I=a.I, E=a.E, n_sims=n, sims_per_set=per, seed=13)
+        jobs.append(native.Job(tables_of(a), rates, x0, **kw))
+        refs.append(oracle_mod.simulate(tables_of(a), rates, x0, **kw))
+    Os, Fs, st = native.batch_simulate(jobs)
+    ev = 0
+    for q, (O, F, (Oc, Fc, sc)) in enumerate(zip(Os, Fs, refs)):
+        print("job", q, O.shape, flush=True)
+        assert np.array_equal(O, Oc), "job %d differs" % q
+        assert np.array_equal(F, Fc)
+        ev += sc["events"]
+    assert st["events"] == ev
+
+
+def test_persistent_refill_many_sims_per_lane(native, oracle_mod):
+    """More simulations than resident lanes: lanes refill from the work queue."""
+    a = models.make_args(models.birth_death(), E=[0], t_max=5.0, ita=6, U=64)
+    n = 300000
+    kw = dict(method=2, t_max=5.0, I=a.I, E=a.E, n_sims=n, seed=21)
+    O, F, st = native.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, **kw)
+    Oc, Fc, sc = oracle_mod.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, **kw)
+    assert np.array_equal(O, Oc)
+    assert st["events"] == sc["events"]
