@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define PST_ABI_VERSION 1
+#define PST_ABI_VERSION 2
 
 /* status codes */
 #define PST_OK 0
@@ -94,6 +94,12 @@ typedef struct pst_config {
   int32_t sims_per_set;       /* simulations sharing one rates/x0 row        */
   uint64_t seed;              /* Philox key                                  */
   uint64_t sim_offset;        /* global id of simulation 0 (Philox counter)  */
+  int32_t specialize;         /* 1: compile a kernel for this network at plan
+                                 creation (hiprtc; cached per network and
+                                 settings), as the reference compiles each
+                                 model's kernels (cuTauLeaping.py:34-35);
+                                 0: table-driven kernel of the library      */
+  int32_t reserved;           /* must be 0                                   */
 } pst_config;
 
 typedef struct pst_stats {

@@ -61,6 +61,8 @@ class PstConfig(ctypes.Structure):
         ("sims_per_set", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
         ("sim_offset", ctypes.c_uint64),
+        ("specialize", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -179,7 +181,7 @@ class _Marshalled:
     """Keeps the numpy arrays behind a PstNetwork/PstConfig alive."""
 
     def __init__(self, tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I, E,
-                 n_sims, sims_per_set, seed, sim_offset):
+                 n_sims, sims_per_set, seed, sim_offset, specialize=False):
         N, M = int(tables["N"]), int(tables["M"])
         self.A = np.ascontiguousarray(tables["A"], np.int32).reshape(-1, 4)
         self.V = np.ascontiguousarray(tables["V"], np.int32).reshape(-1, 4)
@@ -205,16 +207,20 @@ class _Marshalled:
         self.cfg = PstConfig(int(method), int(precision), int(n_c), int(ssa_burst), float(eta),
                              float(t_max), self.I.shape[0], self.E.shape[0],
                              _ptr(self.I, ctypes.c_float), _ptr(self.E, ctypes.c_int32),
-                             int(n_sims), int(sims_per_set), int(seed), int(sim_offset))
+                             int(n_sims), int(sims_per_set), int(seed), int(sim_offset),
+                             1 if specialize else 0, 0)
 
 
 def simulate(tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64, n_c=10,
              eta=0.03, ssa_burst=100, t_max=1.0, I=None, E=(0,), n_sims=1, sims_per_set=None,
-             seed=0, sim_offset=0, return_flags=True):
-    """Run simulations on the GPU; returns (O[kappa][ita][n_sims] int32, flags, stats)."""
+             seed=0, sim_offset=0, return_flags=True, specialize=False):
+    """Run simulations on the GPU; returns (O[kappa][ita][n_sims] int32, flags, stats).
+
+    specialize=True compiles (once per network and settings, cached) a kernel
+    with the network built in instead of using the table-driven kernel."""
     L = load()
     m = _Marshalled(tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I, E,
-                    n_sims, sims_per_set, seed, sim_offset)
+                    n_sims, sims_per_set, seed, sim_offset, specialize)
     O = np.empty((m.E.shape[0], m.I.shape[0], int(n_sims)), np.int32)
     flags = np.zeros(max(int(n_sims), 1), np.int32) if return_flags else None
     st = PstStats()
@@ -231,10 +237,10 @@ class Plan:
 
     def __init__(self, tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64,
                  n_c=10, eta=0.03, ssa_burst=100, t_max=1.0, I=None, E=(0,), n_sims=1,
-                 sims_per_set=None, seed=0, sim_offset=0):
+                 sims_per_set=None, seed=0, sim_offset=0, specialize=False):
         L = load()
         self._m = _Marshalled(tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I,
-                              E, n_sims, sims_per_set, seed, sim_offset)
+                              E, n_sims, sims_per_set, seed, sim_offset, specialize)
         self.n_sims = int(n_sims)
         self.kappa = self._m.E.shape[0]
         self.ita = self._m.I.shape[0]
@@ -279,9 +285,9 @@ class Job:
 
     def __init__(self, tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64, n_c=10,
                  eta=0.03, ssa_burst=100, t_max=1.0, I=None, E=(0,), n_sims=1, sims_per_set=None, seed=0,
-                 sim_offset=0):
+                 sim_offset=0, specialize=False):
         self.m = _Marshalled(tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I, E, n_sims,
-                             sims_per_set, seed, sim_offset)
+                             sims_per_set, seed, sim_offset, specialize)
         self.n_sims = int(n_sims)
         self.kappa = self.m.E.shape[0]
         self.ita = self.m.I.shape[0]

@@ -6,8 +6,10 @@
 // and every branch on it is a scalar branch.  Per-simulation data (state, time,
 // sample cursor, Philox counter) lives in VGPRs for the whole run.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <stdint.h>
 #include <hip/hip_runtime.h>
+#endif
 
 namespace pst {
 
@@ -67,6 +69,8 @@ struct JobDev {
 // SIMD their latency is not hidden.
 template <int NP, int MP>
 struct JobRef {
+  static constexpr int kNP = NP, kMP = MP;
+  using Dev = JobDev<NP, MP>;
   static constexpr int NW = (NP * MP + 3) / 4;
   static constexpr int NS = (NP + 3) / 4;
   uint32_t v[NW], ord[NW], stp[NW];
@@ -115,6 +119,40 @@ struct JobRef {
   __device__ __forceinline__ int Ev(int q) const { return sbyte(E, q); }
 };
 
+// Kernel-side view of a job whose network is compiled into the kernel (the
+// library generates `Net` and compiles it at run time, as the reference
+// compiles each model's propensity code, cuTauLeaping.py:34-35): every table
+// entry and every network-level setting is a constant, so the loops over
+// species and reactions become straight-line code.  Net provides
+//   static constexpr int N, M, kappa, n_c, method, ssa_burst;
+//   static constexpr double eta;
+//   static constexpr int8_t v[N*M], ord[N*M], stp[N*M], H[N], Ht[N], irs[N], E[N];
+// The job descriptor is the JobParams block alone.
+template <class Net>
+struct FixedParams : JobParams {
+  static constexpr int N = Net::N, M = Net::M;
+  static constexpr int kappa = Net::kappa, n_c = Net::n_c, method = Net::method, ssa_burst = Net::ssa_burst;
+  static constexpr double eta = Net::eta;
+};
+
+template <class Net>
+struct FixedRef {
+  static constexpr int kNP = Net::N, kMP = Net::M;
+  using Dev = JobParams;
+  FixedParams<Net> P;
+  __device__ __forceinline__ void load(const JobParams* jp) { static_cast<JobParams&>(P) = *jp; }
+  __device__ __forceinline__ void refresh() {}
+  __device__ __forceinline__ const FixedParams<Net>& p() const { return P; }
+  __device__ __forceinline__ static constexpr int V(int i, int j) { return Net::v[i * kMP + j]; }
+  __device__ __forceinline__ static constexpr int Ord(int i, int j) { return Net::ord[i * kMP + j]; }
+  __device__ __forceinline__ static constexpr int Stp(int i, int j) { return Net::stp[i * kMP + j]; }
+  __device__ __forceinline__ static constexpr int Hv(int i) { return Net::H[i]; }
+  __device__ __forceinline__ static constexpr int Htv(int i) { return Net::Ht[i]; }
+  __device__ __forceinline__ static constexpr int Irs(int i) { return Net::irs[i]; }
+  __device__ __forceinline__ static constexpr int Ev(int q) { return Net::E[q]; }
+};
+
+#ifndef __HIPCC_RTC__
 // host-side copy of a network in dense form, built by the C-ABI layer
 struct HostNet {
   int N, M, kappa;
@@ -166,5 +204,7 @@ inline int size_class(int N, int M, int kappa) {
     if (N <= CLASS_NP[c] && M <= CLASS_MP[c] && kappa <= CLASS_NP[c]) return c;
   return -1;
 }
+
+#endif  // !__HIPCC_RTC__
 
 }  // namespace pst

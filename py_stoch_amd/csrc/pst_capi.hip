@@ -15,8 +15,11 @@
 #include <new>
 #include <vector>
 
+#include <string>
+
 #include "../../include/pst.h"
 #include "pst_args.hpp"
+#include "pst_rtc.hpp"
 
 namespace {
 
@@ -145,7 +148,8 @@ int check_device(int* cu_count) {
 struct PlanJob {
   pst::HostNet net;
   pst::JobParams p;  // O / o_stride / flags filled at execute time
-  int cls = -1;
+  int cls = -1;      // size class of the table-driven kernel, or -1
+  int fixed = -1;    // group of the specialised kernel, or -1
   double* d_rates = nullptr;
   int32_t* d_x0 = nullptr;
   float* d_I = nullptr;
@@ -169,6 +173,15 @@ void pack_jobs(const std::vector<PlanJob>& jobs, int cls, std::vector<unsigned c
 
 }  // namespace
 
+// jobs sharing one specialised kernel (same network and constant settings)
+struct FixedGroup {
+  pst::RtcKernel k;
+  std::vector<int> jobs;
+  void* d_jobs = nullptr;
+  size_t d_bytes = 0;
+  std::vector<unsigned char> uploaded;
+};
+
 struct pst_plan {
   int device = -1;
   int precision = 0;
@@ -177,6 +190,7 @@ struct pst_plan {
   void* d_jobs[pst::NUM_CLASSES] = {nullptr, nullptr, nullptr};
   size_t d_jobs_bytes[pst::NUM_CLASSES] = {0, 0, 0};
   std::vector<unsigned char> uploaded[pst::NUM_CLASSES];  // last descriptors copied to d_jobs
+  std::vector<FixedGroup> fixed;
   unsigned int* d_queue = nullptr;
   unsigned long long* d_stats = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -203,6 +217,8 @@ void pst_plan_destroy(pst_plan* p) {
   }
   for (int c = 0; c < pst::NUM_CLASSES; ++c)
     if (p->d_jobs[c]) (void)hipFree(p->d_jobs[c]);
+  for (FixedGroup& g : p->fixed)
+    if (g.d_jobs) (void)hipFree(g.d_jobs);
   if (p->d_queue) (void)hipFree(p->d_queue);
   if (p->d_stats) (void)hipFree(p->d_stats);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
@@ -231,9 +247,16 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
     if (J.cfg.precision != jobs[0].cfg.precision)
       return fail(PST_EINVAL, "job %d has precision %d, job 0 has %d: one precision per batch", q,
                   J.cfg.precision, jobs[0].cfg.precision);
-    if (pst::size_class(nets[q].N, nets[q].M, nets[q].kappa) < 0)
+    if (J.cfg.specialize != 0 && J.cfg.specialize != 1) return fail(PST_EINVAL, "job %d: specialize must be 0 or 1", q);
+    if (J.cfg.reserved != 0) return fail(PST_EINVAL, "job %d: reserved config field must be 0", q);
+    if (J.cfg.specialize) {
+      if (nets[q].N > pst::RTC_MAX_SPECIES || nets[q].M > pst::RTC_MAX_REACTIONS)
+        return fail(PST_ETOOLARGE, "job %d: N=%d, M=%d is too large for a specialised kernel (N<=%d, M<=%d)", q,
+                    nets[q].N, nets[q].M, pst::RTC_MAX_SPECIES, pst::RTC_MAX_REACTIONS);
+    } else if (pst::size_class(nets[q].N, nets[q].M, nets[q].kappa) < 0) {
       return fail(PST_ETOOLARGE, "job %d: no compiled kernel for N=%d, M=%d, kappa=%d", q, nets[q].N, nets[q].M,
                   nets[q].kappa);
+    }
   }
   int cu = 0;
   int rc = check_device(&cu);
@@ -249,7 +272,7 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
     const pst_job& J = jobs[q];
     PlanJob& pj = p->jobs[q];
     pj.net = nets[q];
-    pj.cls = pst::size_class(pj.net.N, pj.net.M, pj.net.kappa);
+    pj.cls = J.cfg.specialize ? -1 : pst::size_class(pj.net.N, pj.net.M, pj.net.kappa);
     const int N = J.net.n_species, M = J.net.n_reactions;
     const int64_t sets = J.cfg.n_sims == 0 ? 1 : ((int64_t)J.cfg.n_sims + J.cfg.sims_per_set - 1) / J.cfg.sims_per_set;
     if ((e = hipMalloc(&pj.d_rates, sizeof(double) * sets * M)) != hipSuccess ||
@@ -283,6 +306,23 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
     P.rates = pj.d_rates;
     P.x0 = pj.d_x0;
     P.I = pj.d_I;
+    if (J.cfg.specialize) {
+      pst::RtcKernel k;
+      std::string err;
+      const int rc2 = pst::rtc_kernel(pst::rtc_source(pj.net, P, p->precision), p->precision, &k, &err);
+      if (rc2 != PST_OK) {
+        pst_plan_destroy(p);
+        return fail(rc2, "job %d: %s", q, err.c_str());
+      }
+      int g = 0;
+      while (g < (int)p->fixed.size() && p->fixed[g].k.fn != k.fn) ++g;
+      if (g == (int)p->fixed.size()) {
+        p->fixed.emplace_back();
+        p->fixed[g].k = k;
+      }
+      p->fixed[g].jobs.push_back(q);
+      pj.fixed = g;
+    }
   }
   if ((e = hipMalloc(&p->d_queue, sizeof(unsigned int) * n_jobs)) != hipSuccess ||
       (e = hipMalloc(&p->d_stats, sizeof(unsigned long long) * pst::ST_COUNT)) != hipSuccess) {
@@ -352,6 +392,25 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
       return hip_fail(e, "hipMemcpy(jobs)");
     p->uploaded[c] = bytes;
   }
+  // parameter blocks of the specialised kernels
+  for (FixedGroup& g : p->fixed) {
+    const int cnt = (int)g.jobs.size();
+    std::vector<unsigned char> bytes(sizeof(pst::JobParams) * cnt);
+    for (int i = 0; i < cnt; ++i)
+      std::memcpy(bytes.data() + i * sizeof(pst::JobParams), &p->jobs[g.jobs[i]].p, sizeof(pst::JobParams));
+    if (bytes == g.uploaded) continue;
+    if (g.d_bytes < bytes.size()) {
+      if (g.d_jobs) (void)hipFree(g.d_jobs);
+      g.d_jobs = nullptr;
+      g.d_bytes = 0;
+      if ((e = hipMalloc(&g.d_jobs, bytes.size())) != hipSuccess) return hip_fail(e, "hipMalloc(jobs)");
+      g.d_bytes = bytes.size();
+    }
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    if ((e = hipMemcpy(g.d_jobs, bytes.data(), bytes.size(), hipMemcpyHostToDevice)) != hipSuccess)
+      return hip_fail(e, "hipMemcpy(jobs)");
+    g.uploaded = bytes;
+  }
   hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, s, p->d_stats, (int)pst::ST_COUNT);
   hipLaunchKernelGGL(zero_u32, dim3((n_jobs + 255) / 256), dim3(256), 0, s, p->d_queue, n_jobs);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "zero kernels");
@@ -372,6 +431,17 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
     if (rc < 0) return fail(PST_ETOOLARGE, "no compiled kernel for size class %d", c);
     if (rc != 0) return hip_fail((hipError_t)rc, "simulation kernel launch");
     qoff += count[c];
+  }
+  for (FixedGroup& g : p->fixed) {
+    int64_t need = 0;
+    for (int q : g.jobs) need += ((int64_t)p->jobs[q].p.n_sims + 63) / 64;
+    const int64_t resident = (int64_t)g.k.blocks_per_cu * p->cu_count * 4;
+    const int waves = (int)(need < resident ? need : resident);
+    const int cnt = (int)g.jobs.size();
+    if ((e = pst::rtc_launch(g.k, (const pst::JobParams*)g.d_jobs, cnt, p->d_queue + qoff, p->d_stats, waves, s)) !=
+        hipSuccess)
+      return hip_fail(e, "specialised kernel launch");
+    qoff += cnt;
   }
   if ((e = hipEventRecord(p->ev1, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
   p->executed = true;

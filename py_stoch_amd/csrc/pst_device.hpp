@@ -13,8 +13,17 @@
 // translation unit).  The FP32 functions are for the mixed-precision kernels
 // and use the hardware transcendental units.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#else
+// hiprtc (run-time compiled network kernels) declares the fixed-width types here
+using __hip_internal::int8_t;
+using __hip_internal::int32_t;
+using __hip_internal::int64_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
+#endif
 
 namespace pst {
 

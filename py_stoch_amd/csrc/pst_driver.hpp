@@ -9,14 +9,14 @@
 // until the job's queue is empty.  When the job is exhausted and every lane of
 // the wavefront has finished, the wavefront moves to the next job.
 //
-// Per lane the driver keeps a mode -- DECIDE (the P1_P2 body: leap decision
-// and the leap if chosen, P1_P2.py:96-294), SSA (inside a P3 burst,
-// P3.py:71-151) or DONE -- and every pass of its loop runs one DECIDE step for
-// the lanes that need it, then SSA steps for the lanes in a burst: the whole
-// burst if no lane of the wavefront waits to decide, else `ssa_slice` steps.
-// Every simulation executes exactly the reference's sequence (decision, burst
-// of ssa_burst SSA steps, decision, ...); only the interleaving of different
-// simulations within a wavefront changes.
+// Per lane the driver keeps a mode -- DECIDE (next step is a leap decision,
+// P1_P2.py:96-294), SSA (inside a P3 burst, P3.py:71-151, or pure SSA) or
+// DONE.  Every pass of its loop runs one step for every active lane (the
+// engine's step() serves both modes with shared code), then, if at least
+// `ssa_lanes_min` lanes are in an SSA burst, up to `ssa_slice` further SSA-only
+// steps for them.  Every simulation executes exactly the reference's sequence
+// (decision, burst of ssa_burst SSA steps, decision, ...); only the
+// interleaving of different simulations within a wavefront changes.
 #pragma once
 #include "pst_device.hpp"
 #include "pst_args.hpp"
@@ -49,25 +49,23 @@ __device__ __forceinline__ int32_t pick_i(const int32_t (&x)[NP], int idx) {
 }
 
 // Engine interface (see EngineF64 / EngineF32):
-//   int  start(job, sim)          -> initial mode, state loaded
-//   int  decide(job, cnt)         -> next mode (DONE may set flag)
-//   int  ssa_step(job, cnt)       -> 0 continue, 1 ended, 2 overflow
-//   void stop_at_tmax(job, cnt)   -> pure SSA reached t_max
-//   bool before_tmax(job)         -> t < t_max
+//   int  start(job, sim)            -> initial mode, state loaded
+//   int  step<SSA_ONLY>(job, cnt, m) -> next mode (DONE may set flag)
 //   int  flag
-template <class Engine, int NP, int MP>
-__device__ __forceinline__ void run_driver(const JobDev<NP, MP>* __restrict__ jobs, int n_jobs,
+constexpr int SSA_LANES_MIN = 8;
+
+template <class Engine, class Ref>
+__device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__ jobs, int n_jobs,
                                            unsigned int* __restrict__ queue, unsigned long long* __restrict__ stats) {
   const int lane = (int)(threadIdx.x & 63u);
   Counters cnt = {};
   Engine eng;
   int mode = MODE_DONE;
-  int burst = 0;
   int64_t sim = -1;
   int job = 0;
   bool exhausted = false;
 
-  JobRef<NP, MP> jr;
+  Ref jr;
   jr.load(jobs);
 
   for (;;) {
@@ -88,7 +86,6 @@ __device__ __forceinline__ void run_driver(const JobDev<NP, MP>* __restrict__ jo
         if (s < n_sims) {
           sim = s;
           mode = eng.start(jr, s);
-          burst = 0;
         }
       }
     }
@@ -103,43 +100,34 @@ __device__ __forceinline__ void run_driver(const JobDev<NP, MP>* __restrict__ jo
 
     jr.refresh();
     if (wave_any(mode == MODE_DECIDE)) {
-      if (mode == MODE_DECIDE) {
-        mode = eng.decide(jr, cnt);
-        if (mode == MODE_SSA) burst = jr.p().ssa_burst;
+      if (mode != MODE_DONE) {
+        mode = eng.template step<false>(jr, cnt, mode);
         if (mode == MODE_DONE) {
           int32_t* fl = jr.p().flags;
           if (fl) fl[sim] = eng.flag;
           cnt.flagged += (eng.flag != 0);
         }
       }
-    }
-    const int slice0 = jr.p().ssa_slice;
-    const int slice = slice0 > 0 ? slice0 : 1;
-    const int limit = wave_any(mode == MODE_DECIDE) ? slice : 0x7fffffff;
-    for (int s = 0; s < limit; ++s) {
-      if (!wave_any(mode == MODE_SSA)) break;
-      jr.refresh();
-      if (mode == MODE_SSA) {
-        const int method = jr.p().method;
-        if (method == 2 && !eng.before_tmax(jr)) {
-          // kernel_Gillespie stops at t >= t_max; later samples stay zero (cuGillespie.py:71)
-          eng.stop_at_tmax(jr, cnt);
-          mode = MODE_DONE;
-        } else {
-          const int r = eng.ssa_step(jr, cnt);
-          if (r != 0) {
-            mode = MODE_DONE;
-            if (r == 2) eng.flag = FLAG_OVERFLOW;
-          } else if (method == 0) {
-            burst -= 1;
-            if (burst <= 0) mode = MODE_DECIDE;
+      // SSA-only steps while enough lanes are inside a burst
+      const int slice = jr.p().ssa_slice;
+      for (int s = 0; s < slice; ++s) {
+        if (__builtin_popcountll(__builtin_amdgcn_ballot_w64(mode == MODE_SSA)) < SSA_LANES_MIN) break;
+        jr.refresh();
+        if (mode == MODE_SSA) {
+          mode = eng.template step<true>(jr, cnt, mode);
+          if (mode == MODE_DONE) {
+            int32_t* fl = jr.p().flags;
+            if (fl) fl[sim] = eng.flag;
+            cnt.flagged += (eng.flag != 0);
           }
         }
-        if (mode == MODE_DONE) {
-          int32_t* fl = jr.p().flags;
-          if (fl) fl[sim] = eng.flag;
-          cnt.flagged += (eng.flag != 0);
-        }
+      }
+    } else if (mode == MODE_SSA) {
+      mode = eng.template step<true>(jr, cnt, mode);
+      if (mode == MODE_DONE) {
+        int32_t* fl = jr.p().flags;
+        if (fl) fl[sim] = eng.flag;
+        cnt.flagged += (eng.flag != 0);
       }
     }
   }

@@ -1,0 +1,503 @@
+// pst_engine_f32.hpp -- mixed FP32/FP64 simulation engine for gfx950.
+//
+// Same method as the FP64 kernels (P1_P2.py:54-297, P3.py:29-163,
+// gillespie.py:27-149): same epsilon bound, same n_c critical split, same
+// SSA fallback burst and sampling rules, same wavefront-uniform loop.
+// Precision:
+//   FP32: propensities, a0 for the leap decision, mu / sigma^2, the Cao tau
+//         bound, Poisson means and the whole Poisson sampler (PTRS log-density
+//         test in a cancellation-free Stirling form), uniforms (24-bit, one
+//         Philox word each)
+//   FP64: the simulation clock t, and the reaction choice, done as "first j
+//         with sum_{r<=j} a_r >= rho2 * a0" accumulated in FP64 so that FP32
+//         round-off cannot bias which reaction fires.  Poisson means of 2^22
+//         and above use the FP64 sampler (FP32 cannot resolve the integers).
+// Transcendentals use the hardware v_log_f32 / v_exp_f32 / v_rcp_f32.  The
+// launch loop, work queue and lane modes are in pst_driver.hpp.
+#pragma once
+#include "pst_driver.hpp"
+
+namespace pst {
+
+template <class Ref, int NP = Ref::kNP, int MP = Ref::kMP>
+__device__ __forceinline__ void propensities_f32(const Ref& tab, const float (&k)[MP],
+                                                 const int32_t (&x)[NP], float (&a)[MP]) {
+#pragma unroll
+  for (int j = 0; j < MP; ++j) {
+    float v = k[j];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const int p = tab.Ord(i, j);
+      const int st = tab.Stp(i, j);
+      for (int q = 0; q < p; ++q) v = v * (float)(x[i] - st * q);
+    }
+    a[j] = v;
+  }
+}
+
+// CalculateG (P1_P2.py:447-486) in FP32
+__device__ __forceinline__ float calc_g_f32(int h, int ht, int xi) {
+  const float xf = (float)xi;
+  if (h == 1) return 1.0f;
+  if (h == 2) return (ht == 2) ? 2.0f + fast_rcp(xf - 1.0f) : 2.0f;
+  if (h == 3) {
+    if (ht == 3) return 3.0f + fast_rcp(xf - 1.0f) + 2.0f * fast_rcp(xf - 2.0f);
+    if (ht == 2) return 1.5f * (2.0f + fast_rcp(xf - 1.0f));
+    return 3.0f;
+  }
+  return 0.0f;
+}
+
+template <int NP, typename T>
+__device__ __forceinline__ T pick_m(const T (&x)[NP], int idx) {
+  T r = x[0];
+#pragma unroll
+  for (int i = 1; i < NP; ++i)
+    if (idx == i) r = x[i];
+  return r;
+}
+
+// log(1 + d) with the rounding of 1 + d corrected (Goldberg's method)
+__device__ __forceinline__ float log1p_f32(float d) {
+  const float u = 1.0f + d;
+  const float m = u - 1.0f;
+  return (m == 0.0f) ? d : fast_log(u) * (d / m);
+}
+
+struct PtrsF32 {
+  float lam, slam, b, a, vr, twoa;
+};
+
+__device__ __forceinline__ PtrsF32 ptrs_setup_f32(float lam) {
+  PtrsF32 P;
+  P.lam = lam;
+  P.slam = __builtin_sqrtf(lam);
+  P.b = 0.931f + 2.53f * P.slam;
+  P.a = -0.059f + 0.02483f * P.b;
+  P.vr = 0.9277f - 3.6224f * fast_rcp(P.b - 2.0f);
+  P.twoa = 2.0f * P.a;
+  return P;
+}
+
+__device__ __forceinline__ bool ptrs_fast_f32(const PtrsF32& P, float U, float V, float& kf, float& us) {
+  us = 0.5f - __builtin_fabsf(U);
+  kf = __builtin_floorf((P.twoa * fast_rcp(us) + P.b) * U + P.lam + 0.43f);
+  return us >= 0.07f && V <= P.vr;
+}
+
+// -lambda + k ln lambda - ln k!, the right-hand side of PTRS's acceptance test.
+// For k >= 10, with x = k + 1 and Stirling's series for ln Gamma(x):
+//   (x - lambda) - (x - 1/2) log1p((x - lambda)/lambda) - ln(lambda)/2
+//   - ln(2 pi)/2 - 1/(12x) + 1/(360x^3)
+// which keeps full FP32 accuracy where the direct form cancels.
+__device__ __forceinline__ float log_pmf_f32(float kf, float lam) {
+  if (kf < 10.0f) {
+    const int k = (int)kf;
+    const float lf = k < 2 ? 0.0f : (float)logfact_small(k);
+    return -lam + kf * fast_log(lam) - lf;
+  }
+  const float x = kf + 1.0f;
+  const float d = x - lam;
+  const float r = fast_rcp(x);
+  const float ser = r * (1.0f / 12.0f - r * r * (1.0f / 360.0f));
+  return d - (x - 0.5f) * log1p_f32(d * fast_rcp(lam)) - 0.5f * fast_log(lam) - 0.91893853320467274f - ser;
+}
+
+__device__ __noinline__ bool ptrs_slow_f32(const PtrsF32& P, float V, float kf, float us) {
+  if (kf < 0.0f || (us < 0.013f && V > us)) return false;
+  const float invalpha = 1.1239f + 1.1328f * fast_rcp(P.b - 3.4f);
+  const float lhs = fast_log(V) + fast_log(invalpha) - fast_log(P.a * fast_rcp(us * us) + P.b);
+  return lhs <= log_pmf_f32(kf, P.lam);
+}
+
+// PTRS iterations it0, it0 + 1, ... of the draw whose iteration `it` uses the
+// words of half h of block (c, it); -1 at the iteration limit.
+__device__ __noinline__ int64_t ptrs_rest_f32(uint32_t k0, uint32_t k1, uint32_t s_lo, uint32_t s_hi, uint64_t c,
+                                              int h, int it0, float lam, unsigned int* wptrs) {
+  Key key;
+  key.k0 = k0;
+  key.k1 = k1;
+  key.s_lo = s_lo;
+  key.s_hi = s_hi;
+  const PtrsF32 P = ptrs_setup_f32(lam);
+  for (int it = it0; it < PTRS_MAX_ITERS; ++it) {
+    const Words4 bw = key.block(c, (uint32_t)it);
+    *wptrs += 1;
+    const float U = u24(h ? bw.w2 : bw.w0) - 0.5f;
+    const float V = u24(h ? bw.w3 : bw.w1);
+    float kf, us;
+    if (ptrs_fast_f32(P, U, V, kf, us)) return (int64_t)kf;
+    if (ptrs_slow_f32(P, V, kf, us)) return (int64_t)kf;
+  }
+  return -1;
+}
+
+// Poisson means too large for FP32 integer resolution: PTRS in FP64
+__device__ __noinline__ int64_t ptrs_big_f64(uint32_t k0, uint32_t k1, uint32_t s_lo, uint32_t s_hi, uint64_t c,
+                                             int h, double lam, unsigned int* wptrs) {
+  Key key;
+  key.k0 = k0;
+  key.k1 = k1;
+  key.s_lo = s_lo;
+  key.s_hi = s_hi;
+  const double loglam = log_f64(lam);
+  const double b = 0.931 + 2.53 * __builtin_sqrt(lam);
+  const double a = -0.059 + 0.02483 * b;
+  const double vr = 0.9277 - 3.6224 / (b - 2.0);
+  const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+  for (int it = 0; it < PTRS_MAX_ITERS; ++it) {
+    const Words4 bw = key.block(c, (uint32_t)it);
+    *wptrs += 1;
+    const double U = u53(h ? bw.w2 : bw.w0, h ? bw.w3 : bw.w1) - 0.5;
+    const double V = u53(h ? bw.w3 : bw.w1, h ? bw.w2 : bw.w0);
+    const double us = 0.5 - __builtin_fabs(U);
+    const double kf = __builtin_floor((2.0 * a / us + b) * U + lam + 0.43);
+    if (us >= 0.07 && V <= vr) return (int64_t)kf;
+    if (kf < 0.0 || (us < 0.013 && V > us)) continue;
+    const double lhs = log_f64(V) + log_f64(invalpha) - log_f64(a / (us * us) + b);
+    const double rhs = -lam + kf * loglam - lgamma_int_f64(kf + 1.0);
+    if (lhs <= rhs) return (int64_t)kf;
+  }
+  return -1;
+}
+
+constexpr float BIG_LAMBDA_F32 = 4194304.0f;  // 2^22
+
+// Per-lane state and steps of one mixed-precision simulation.
+template <class Ref>
+struct EngineF32 {
+  static constexpr int NP = Ref::kNP, MP = Ref::kMP;
+  float k[MP];
+  int32_t x[NP];
+  Key key;
+  uint64_t bc;
+  double t, tnext;
+  int F;
+  int flag;
+  int burst;
+  int32_t* Ob;
+
+  __device__ __forceinline__ void fill_rest(const Ref& tab, bool zero, Counters& cnt) const {
+    const auto& P = tab.p();
+    cnt.w.samples += (unsigned int)(P.ita - F);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      if (q < P.kappa) {
+        const int32_t val = zero ? 0 : pick_i<NP>(x, tab.Ev(q));
+        for (int f = F; f < P.ita; ++f) Ob[((int64_t)q * P.ita + f) * P.o_stride] = val;
+      }
+    }
+  }
+
+  __device__ __forceinline__ int start(const Ref& jr, int64_t sim) {
+    const auto& P = jr.p();
+    const int64_t set = sim / P.sims_per_set;
+#pragma unroll
+    for (int j = 0; j < MP; ++j) k[j] = (j < P.M) ? (float)P.rates[set * P.M + j] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) x[i] = (i < P.N) ? P.x0[set * P.N + i] : 0;
+    key.init(P.seed, P.sim_offset + (uint64_t)sim);
+    bc = 0;
+    t = 0.0;
+    F = 0;
+    tnext = (double)P.I[0];
+    flag = 0;
+    burst = 0;
+    Ob = P.O + sim;
+    return P.method == 0 ? MODE_DECIDE : MODE_SSA;
+  }
+
+  // One step: leap decision and leap (P1_P2.py:96-294), or one SSA step
+  // (P3.py:73-149, gillespie.py:70-149) -- see EngineF64::step for the
+  // structure; the arithmetic is FP32 except the clock and the cumulative
+  // sum of the reaction choice.
+  template <bool SSA_ONLY>
+  __device__ __forceinline__ int step(const Ref& tab, Counters& cnt, int mode) {
+    const auto& P = tab.p();
+    const int M = P.M;
+    bool ssa = SSA_ONLY || (mode == MODE_SSA);
+    if (ssa && P.method == 2 && !(t < P.t_max)) {
+      fill_rest(tab, true, cnt);
+      return MODE_DONE;
+    }
+    float a[MP];
+    propensities_f32<Ref>(tab, k, x, a);
+    float a0 = 0.0f;
+#pragma unroll
+    for (int j = 0; j < MP; ++j) a0 += a[j];
+    if (a0 == 0.0f) {
+      fill_rest(tab, false, cnt);
+      return MODE_DONE;
+    }
+    bool crit[MP];
+#pragma unroll
+    for (int j = 0; j < MP; ++j) crit[j] = false;
+    float tau1 = __builtin_inff();
+    if (!SSA_ONLY && !ssa) {
+      cnt.w.decisions += 1;
+      // critical reactions (P1_P2.py:314-327)
+#pragma unroll
+      for (int j = 0; j < MP; ++j) {
+        bool c = false;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+          const int v = tab.V(i, j);
+          if (v < 0) c = c | ((-v) * P.n_c > x[i]);
+        }
+        crit[j] = c;
+      }
+      const float eta = (float)P.eta;
+#pragma unroll
+      for (int i = 0; i < NP; ++i) {
+        if (tab.Irs(i)) {
+          float mu = 0.0f, s2 = 0.0f;
+          bool any = false;
+#pragma unroll
+          for (int j = 0; j < MP; ++j) {
+            const int v = tab.V(i, j);
+            if (v != 0) {
+              const bool nc = !crit[j];
+              mu += nc ? (float)v * a[j] : 0.0f;
+              s2 += nc ? (float)(v * v) * a[j] : 0.0f;
+              any = any | nc;
+            }
+          }
+          const float g_i = calc_g_f32(tab.Hv(i), tab.Htv(i), x[i]);
+          const float num = fmaxf(eta * (float)x[i] * fast_rcp(g_i), 1.0f);
+          const float cand = fminf(num * fast_rcp(fabsf(mu)), num * num * fast_rcp(s2));
+          tau1 = any ? fminf(tau1, cand) : tau1;
+        }
+      }
+      if (tau1 < 10.0f * fast_rcp(a0)) {
+        ssa = true;
+        burst = P.ssa_burst;
+      }
+    }
+    bool msk[MP];
+    double asum = 0.0;
+#pragma unroll
+    for (int j = 0; j < MP; ++j) {
+      msk[j] = ssa ? (j < M) : crit[j];
+      asum += msk[j] ? (double)a[j] : 0.0;
+    }
+    float tauX = __builtin_inff();
+    uint32_t cw1 = 0;
+    if (asum > 0.0) {
+      const Words4 bw = key.block(bc, 0);
+      bc += 1;
+      cnt.w.uniforms += ssa ? 2 : 1;
+      cw1 = bw.w1;
+      tauX = -fast_log(u24(bw.w0)) * fast_rcp((float)asum);
+    }
+    const float tau_f = ssa ? tauX : fminf(tau1, tauX);
+    int sel = -1;
+    if (ssa || tau1 > tauX) {
+      if (!ssa) cnt.w.uniforms += 1;
+      // first j of the mask with sum_{r<=j} a_r >= rho2 * asum, summed in FP64
+      const double target = (double)u24(cw1) * asum;
+      double cum = 0.0;
+      int last = -1;
+#pragma unroll
+      for (int j = 0; j < MP; ++j) {
+        if (msk[j] && sel < 0) {
+          if (a[j] > 0.0f) last = j;
+          cum += (double)a[j];
+          if (cum >= target) sel = j;
+        }
+      }
+      if (sel < 0) sel = last;
+    }
+    int32_t xn[NP];
+    float tau = tau_f;
+    if (ssa) {
+#pragma unroll
+      for (int i = 0; i < NP; ++i) {
+        int dx = 0;
+#pragma unroll
+        for (int j = 0; j < MP; ++j)
+          if (sel == j) dx = tab.V(i, j);
+        xn[i] = x[i] + dx;
+      }
+    } else if (!SSA_ONLY) {
+      if (!(tau < __builtin_inff())) {
+        fill_rest(tab, true, cnt);
+        flag = FLAG_TAU_INF;
+        return MODE_DONE;
+      }
+      int32_t K[MP];
+#pragma unroll
+      for (int j = 0; j < MP; ++j) K[j] = (sel == j) ? 1 : 0;
+      const int st = leap(tab, cnt, a, crit, tau, K, xn);
+      if (st != 0) {
+        fill_rest(tab, true, cnt);
+        flag = st;
+        return MODE_DONE;
+      }
+      unsigned long long ev = 0;
+#pragma unroll
+      for (int j = 0; j < MP; ++j) ev += (unsigned long long)K[j];
+      cnt.events += ev;
+      cnt.leaps += 1;
+    }
+    const double t0 = t;
+    t = t + (double)tau;
+    if (t >= tnext) {
+      const float inv_gap = fast_rcp((float)(t - t0));
+      while (t >= tnext) {
+        // sample-and-hold for an SSA step: a zero fraction keeps the pre-step state
+        const float frac = ssa ? 0.0f : (float)(tnext - t0) * inv_gap;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+          if (q < P.kappa) {
+            const int s = tab.Ev(q);
+            const int32_t xs = pick_i<NP>(x, s);
+            const int diff = pick_i<NP>(xn, s) - xs;
+            Ob[((int64_t)q * P.ita + F) * P.o_stride] = xs + rn_f32_to_int((float)diff * frac);
+          }
+        }
+        cnt.w.samples += 1;
+        F = F + 1;
+        if (F == P.ita) return MODE_DONE;
+        tnext = (double)P.I[F];
+      }
+    }
+    if (ssa) {
+      bool over = false;
+#pragma unroll
+      for (int i = 0; i < NP; ++i) over = over | (xn[i] >= COUNT_LIMIT);
+      cnt.ssa += 1;
+      cnt.events += 1;
+#pragma unroll
+      for (int i = 0; i < NP; ++i) x[i] = xn[i];
+      if (over) {
+        fill_rest(tab, true, cnt);
+        flag = FLAG_OVERFLOW;
+        return MODE_DONE;
+      }
+      if (P.method != 0) return MODE_SSA;
+      burst -= 1;
+      return burst > 0 ? MODE_SSA : MODE_DECIDE;
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i) x[i] = xn[i];
+    return MODE_DECIDE;
+  }
+
+  // Poisson leap with tau halving (P1_P2.py:222-259,509-540): reactions 2r and
+  // 2r+1 draw from the two halves of block (grp + r, it).  Returns 0 or a flag.
+  __device__ __forceinline__ int leap(const Ref& tab, Counters& cnt,
+                                      const float (&a)[MP], const bool (&crit)[MP], float tau,
+                                      int32_t (&K)[MP], int32_t (&xn)[NP]) {
+    constexpr int NB = (MP + 1) / 2;
+    const auto& P = tab.p();
+    const int M = P.M;
+    const int nblk = (M + 1) >> 1;
+    for (int h = 0; h < MAX_HALVINGS; ++h) {
+      const uint64_t grp = bc;
+      bc += (uint64_t)nblk;
+      Words4 W[NB];
+#pragma unroll
+      for (int r = 0; r < NB; ++r)
+        if (2 * r < M) W[r] = key.block(grp + (uint64_t)r, 0);
+      float lam[MP];
+      unsigned pend = 0, inv = 0, drawn = 0;
+      bool bad = false;
+#pragma unroll
+      for (int j = 0; j < MP; ++j) {
+        const float l = (j < M && !crit[j]) ? a[j] * tau : 0.0f;
+        lam[j] = l;
+        const uint32_t wu = (j & 1) ? W[j >> 1].w2 : W[j >> 1].w0;
+        const uint32_t wv = (j & 1) ? W[j >> 1].w3 : W[j >> 1].w1;
+        const bool use = (l >= 10.0f) & (l < BIG_LAMBDA_F32);
+        bool ok = false;
+        float kf = 0.0f;
+        if (wave_any(use)) {
+          // PTRS iteration 0 (squeeze, then the log-density test)
+          const PtrsF32 Pt = ptrs_setup_f32(fmaxf(l, 10.0f));
+          float us;
+          const float V = u24(wv);
+          ok = ptrs_fast_f32(Pt, u24(wu) - 0.5f, V, kf, us);
+          if (use & !ok) ok = ptrs_slow_f32(Pt, V, kf, us);
+          ok = ok & use;
+        }
+        if (!crit[j]) K[j] = ok ? (int32_t)kf : 0;
+        pend |= ((l >= 10.0f) & !ok) ? (1u << j) : 0u;
+        drawn |= use ? (1u << j) : 0u;
+        inv |= ((l > 0.0f) & (l < 10.0f)) ? (1u << j) : 0u;
+        bad = bad | (l >= (float)LAMBDA_LIMIT);
+      }
+      cnt.w.ptrs += (unsigned int)__builtin_popcount(drawn);
+      cnt.w.uniforms += 2u * (unsigned int)__builtin_popcount(drawn);
+      if (bad) return FLAG_OVERFLOW;
+      // sequential-search inversion (0 < lambda < 10) for all such reactions
+      // in lock-step: after `it` trips every live search is at k = it
+      if (inv != 0u) {
+        float p[MP], Fc[MP], u[MP];
+        unsigned act = 0;
+#pragma unroll
+        for (int j = 0; j < MP; ++j) {
+          const bool on = (inv >> j) & 1u;
+          u[j] = u24((j & 1) ? W[j >> 1].w2 : W[j >> 1].w0);
+          p[j] = fast_exp(-lam[j]);
+          Fc[j] = p[j];
+          act |= (on && u[j] > Fc[j]) ? (1u << j) : 0u;
+        }
+        cnt.w.uniforms += (unsigned int)__builtin_popcount(inv);
+        int it = 0;
+        while (act != 0u && it < INV_MAX_K) {
+          it += 1;
+          const float rk = fast_rcp((float)it);
+#pragma unroll
+          for (int j = 0; j < MP; ++j) {
+            const bool on = (act >> j) & 1u;
+            const float pn = p[j] * lam[j] * rk;
+            const float Fn = Fc[j] + pn;
+            p[j] = on ? pn : p[j];
+            Fc[j] = on ? Fn : Fc[j];
+            K[j] = on ? it : K[j];
+            act = (on && !(u[j] > Fn)) ? (act & ~(1u << j)) : act;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < MP; ++j)
+          if ((inv >> j) & 1u) cnt.w.inv += (unsigned int)K[j];
+      }
+      // draws rejected at iteration 0 (and means of 2^22 and above, drawn in FP64)
+      while (pend != 0u) {
+        const int j = __builtin_ctz(pend);
+        pend &= pend - 1u;
+        const float l = pick_m<MP>(lam, j);
+        unsigned int extra = 0;
+        const int64_t kk = (l >= BIG_LAMBDA_F32)
+                               ? ptrs_big_f64(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1,
+                                              (double)l, &extra)
+                               : ptrs_rest_f32(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1,
+                                               1, l, &extra);
+        cnt.w.ptrs += extra;
+        cnt.w.uniforms += 2 * extra;
+        if (kk < 0) return FLAG_PTRS;
+#pragma unroll
+        for (int r = 0; r < MP; ++r) K[r] = (r == j) ? (int32_t)kk : K[r];
+      }
+      bool ok = true, big = false;
+#pragma unroll
+      for (int i = 0; i < NP; ++i) {
+        int64_t sacc = x[i];
+#pragma unroll
+        for (int j = 0; j < MP; ++j) {
+          const int v = tab.V(i, j);
+          if (v != 0) sacc += (int64_t)K[j] * v;
+        }
+        ok = ok & (sacc >= 0);
+        big = big | (sacc >= COUNT_LIMIT);
+        xn[i] = (int32_t)sacc;
+      }
+      if (ok) return big ? FLAG_OVERFLOW : 0;
+      cnt.rejected += 1;
+      tau = tau * 0.5f;
+    }
+    return FLAG_HALVING;
+  }
+};
+
+}  // namespace pst

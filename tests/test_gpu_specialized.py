@@ -1,0 +1,95 @@
+"""Network-specific kernels compiled at run time (specialize=True): the FP64
+kernel gives the same bits as the CPU oracle on the same Philox stream, like
+the table-driven kernel, for every method; jobs of several networks in one
+plan are grouped per network (every test marked gpu)."""
+import numpy as np
+import pytest
+
+from py_stoch_amd import models
+from py_stoch_amd.mod.parser.tl_parser import tables_of
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(native, oracle_mod, spn, E, t_max, ita, U, method, seed=7):
+    a = models.make_args(spn, E=E, t_max=t_max, ita=ita, U=U)
+    tabs = tables_of(a)
+    rates = a.rate_fn(a.c)
+    kw = dict(method=method, t_max=t_max, I=a.I, E=a.E, n_sims=U, seed=seed)
+    print("case: specialised U=%d ita=%d method=%d seed=%d" % (U, ita, method, seed), flush=True)
+    O_g, f_g, s_g = native.simulate(tabs, rates, a.x_0, precision=native.PRECISION_FP64, specialize=True, **kw)
+    O_c, f_c, s_c = oracle_mod.simulate(tabs, rates, a.x_0, **kw)
+    mism = np.argwhere(O_g != O_c)
+    assert mism.shape[0] == 0, "first mismatch %s gpu=%d cpu=%d" % (
+        mism[0].tolist(), O_g[tuple(mism[0])], O_c[tuple(mism[0])])
+    assert np.array_equal(f_g, f_c)
+    for k in ("events", "leaps", "rejected_leaps", "ssa_steps", "flagged"):
+        assert s_g[k] == s_c[k], (k, s_g[k], s_c[k])
+    return O_g, s_g
+
+
+def test_lotka_volterra_tau_leaping(native, oracle_mod):
+    _, st = _compare(native, oracle_mod, models.lotka_volterra(), [0, 1], 10.0, 1000, 2048,
+                     native.METHOD_TAU_LEAPING, seed=2)
+    assert st["leaps"] > 0 and st["ssa_steps"] > 0
+
+
+def test_schlogl_tau_leaping(native, oracle_mod):
+    _compare(native, oracle_mod, models.schlogl(), [0], 10.0, 100, 1024, native.METHOD_TAU_LEAPING, seed=3)
+
+
+def test_birth_death_pure_and_forced_ssa(native, oracle_mod):
+    _compare(native, oracle_mod, models.birth_death(), [0], 100.0, 101, 1024, native.METHOD_PURE_SSA, seed=0)
+    _compare(native, oracle_mod, models.birth_death(), [0], 100.0, 101, 1024, native.METHOD_FORCED_SSA, seed=0)
+
+
+def test_recorded_species_order_and_ragged_counts(native, oracle_mod):
+    for U in (1, 65, 257):
+        _compare(native, oracle_mod, models.lotka_volterra(), [1, 0], 2.0, 17, U, native.METHOD_TAU_LEAPING, seed=U)
+
+
+def test_absorbing_state(native, oracle_mod):
+    spn = models.birth_death(birth=0.0, death=1.0, x0=30)
+    O, _ = _compare(native, oracle_mod, spn, [0], 50.0, 51, 300, native.METHOD_TAU_LEAPING, seed=5)
+    assert (O[0, -1] == 0).all()
+
+
+def test_batch_groups_networks(native, oracle_mod):
+    """Two LV parameter sweeps and one Schlogl sweep in one plan: two compiled
+    kernels, every job bit-identical to its oracle run."""
+    rng = np.random.default_rng(11)
+    jobs, refs = [], []
+    a_lv = models.make_args(models.lotka_volterra(), E=[0, 1], t_max=2.0, ita=21, U=64)
+    a_sc = models.make_args(models.schlogl(), E=[0], t_max=2.0, ita=11, U=64)
+    for a, sets, per, M in ((a_lv, 5, 70, 3), (a_sc, 3, 50, 4), (a_lv, 2, 40, 3)):
+        rates = np.array([a.rate_fn(a.c) * 10.0 ** rng.uniform(-0.3, 0.3, M) for _ in range(sets)])
+        x0 = np.tile(a.x_0, (sets, 1))
+        kw = dict(method=0, t_max=2.0, I=a.I, E=a.E, n_sims=sets * per, sims_per_set=per, seed=17)
+        jobs.append(native.Job(tables_of(a), rates, x0, specialize=True, **kw))
+        refs.append(oracle_mod.simulate(tables_of(a), rates, x0, **kw))
+    Os, Fs, st = native.batch_simulate(jobs)
+    for q, (O, F, (Oc, Fc, _)) in enumerate(zip(Os, Fs, refs)):
+        print("job", q, O.shape, flush=True)
+        assert np.array_equal(O, Oc), "job %d differs" % q
+        assert np.array_equal(F, Fc)
+    assert st["events"] == sum(r[2]["events"] for r in refs)
+
+
+def test_mixed_specialised_matches_table_kernel(native):
+    """The mixed-precision kernel with the network compiled in samples the
+    same law as the table-driven one (constant folding may change FP32
+    contractions, so trajectories are compared in distribution): two-sample
+    KS distance of both species at every fifth sample below the 0.01% critical
+    value for 4096 + 4096 draws."""
+    a = models.make_args(models.lotka_volterra(), E=[0, 1], t_max=5.0, ita=51, U=4096)
+    kw = dict(method=0, t_max=5.0, I=a.I, E=a.E, n_sims=4096, precision=native.PRECISION_MIXED)
+    O_t, _, s_t = native.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, seed=4, **kw)
+    O_s, _, s_s = native.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, seed=5, specialize=True, **kw)
+    crit = 2.23 * np.sqrt(2.0 / 4096)
+    for k in range(2):
+        for f in range(5, 51, 5):
+            x, y = np.sort(O_t[k, f]), np.sort(O_s[k, f])
+            grid = np.concatenate([x, y])
+            d = np.abs(np.searchsorted(x, grid, side="right") - np.searchsorted(y, grid, side="right")).max() / 4096
+            assert d < crit, (k, f, d)
+    assert abs(s_t["events"] / s_s["events"] - 1.0) < 0.02
