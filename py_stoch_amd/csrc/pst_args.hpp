@@ -54,6 +54,19 @@ struct JobParams {
   int32_t* flags;       // [n_sims] or null
 };
 
+// Copy a job's settings into VGPRs (an empty asm on each 32-bit word hides
+// their uniformity).  They are read only by per-lane arithmetic (addresses,
+// sample cursor, queue bounds), and held in SGPRs they would exceed the
+// scalar register file and be spilled and reloaded lane by lane.
+__device__ __forceinline__ void load_params_vgpr(JobParams* dst, const JobParams* src) {
+  constexpr int NWORDS = (int)(sizeof(JobParams) / 4);
+  uint32_t w[NWORDS];
+  __builtin_memcpy(w, src, sizeof(JobParams));
+#pragma unroll
+  for (int i = 0; i < NWORDS; ++i) asm volatile("" : "+v"(w[i]));
+  __builtin_memcpy(dst, w, sizeof(JobParams));
+}
+
 template <int NP, int MP>
 struct JobDev {
   NetTab<NP, MP> tab;
@@ -85,7 +98,7 @@ struct JobRef {
     __builtin_memcpy(Ht, jd->tab.Ht, sizeof(jd->tab.Ht));
     __builtin_memcpy(irs, jd->tab.irs, sizeof(jd->tab.irs));
     __builtin_memcpy(E, jd->tab.E, sizeof(jd->tab.E));
-    P = jd->p;
+    load_params_vgpr(&P, &jd->p);
   }
   static __device__ __forceinline__ uint32_t opaque(uint32_t w) {
     asm volatile("" : "+v"(w));
@@ -140,7 +153,7 @@ struct FixedRef {
   static constexpr int kNP = Net::N, kMP = Net::M;
   using Dev = JobParams;
   FixedParams<Net> P;
-  __device__ __forceinline__ void load(const JobParams* jp) { static_cast<JobParams&>(P) = *jp; }
+  __device__ __forceinline__ void load(const JobParams* jp) { load_params_vgpr(&P, jp); }
   __device__ __forceinline__ void refresh() {}
   __device__ __forceinline__ const FixedParams<Net>& p() const { return P; }
   __device__ __forceinline__ static constexpr int V(int i, int j) { return Net::v[i * kMP + j]; }

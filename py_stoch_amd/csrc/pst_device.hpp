@@ -47,6 +47,13 @@ struct Words4 {
   uint32_t w0, w1, w2, w3;
 };
 
+// result of a Poisson draw finished out of line: the count (-1 at the
+// iteration limit) and the rejection-loop iterations it took
+struct Draw {
+  int64_t k;
+  uint32_t iters;
+};
+
 __device__ __forceinline__ Words4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                                  uint32_t k0, uint32_t k1) {
 #pragma unroll

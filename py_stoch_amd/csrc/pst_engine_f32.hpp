@@ -103,7 +103,7 @@ __device__ __forceinline__ float log_pmf_f32(float kf, float lam) {
   return d - (x - 0.5f) * log1p_f32(d * fast_rcp(lam)) - 0.5f * fast_log(lam) - 0.91893853320467274f - ser;
 }
 
-__device__ __noinline__ bool ptrs_slow_f32(const PtrsF32& P, float V, float kf, float us) {
+__device__ __forceinline__ bool ptrs_slow_f32(const PtrsF32& P, float V, float kf, float us) {
   if (kf < 0.0f || (us < 0.013f && V > us)) return false;
   const float invalpha = 1.1239f + 1.1328f * fast_rcp(P.b - 3.4f);
   const float lhs = fast_log(V) + fast_log(invalpha) - fast_log(P.a * fast_rcp(us * us) + P.b);
@@ -112,29 +112,32 @@ __device__ __noinline__ bool ptrs_slow_f32(const PtrsF32& P, float V, float kf, 
 
 // PTRS iterations it0, it0 + 1, ... of the draw whose iteration `it` uses the
 // words of half h of block (c, it); -1 at the iteration limit.
-__device__ __noinline__ int64_t ptrs_rest_f32(uint32_t k0, uint32_t k1, uint32_t s_lo, uint32_t s_hi, uint64_t c,
-                                              int h, int it0, float lam, unsigned int* wptrs) {
+__device__ __noinline__ Draw ptrs_rest_f32(uint32_t k0, uint32_t k1, uint32_t s_lo, uint32_t s_hi, uint64_t c, int h,
+                                           int it0, float lam) {
   Key key;
   key.k0 = k0;
   key.k1 = k1;
   key.s_lo = s_lo;
   key.s_hi = s_hi;
   const PtrsF32 P = ptrs_setup_f32(lam);
+  Draw d = {-1, 0u};
   for (int it = it0; it < PTRS_MAX_ITERS; ++it) {
     const Words4 bw = key.block(c, (uint32_t)it);
-    *wptrs += 1;
+    d.iters += 1;
     const float U = u24(h ? bw.w2 : bw.w0) - 0.5f;
     const float V = u24(h ? bw.w3 : bw.w1);
     float kf, us;
-    if (ptrs_fast_f32(P, U, V, kf, us)) return (int64_t)kf;
-    if (ptrs_slow_f32(P, V, kf, us)) return (int64_t)kf;
+    if (ptrs_fast_f32(P, U, V, kf, us) || ptrs_slow_f32(P, V, kf, us)) {
+      d.k = (int64_t)kf;
+      break;
+    }
   }
-  return -1;
+  return d;
 }
 
 // Poisson means too large for FP32 integer resolution: PTRS in FP64
-__device__ __noinline__ int64_t ptrs_big_f64(uint32_t k0, uint32_t k1, uint32_t s_lo, uint32_t s_hi, uint64_t c,
-                                             int h, double lam, unsigned int* wptrs) {
+__device__ __noinline__ Draw ptrs_big_f64(uint32_t k0, uint32_t k1, uint32_t s_lo, uint32_t s_hi, uint64_t c, int h,
+                                          double lam) {
   Key key;
   key.k0 = k0;
   key.k1 = k1;
@@ -145,20 +148,27 @@ __device__ __noinline__ int64_t ptrs_big_f64(uint32_t k0, uint32_t k1, uint32_t 
   const double a = -0.059 + 0.02483 * b;
   const double vr = 0.9277 - 3.6224 / (b - 2.0);
   const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+  Draw d = {-1, 0u};
   for (int it = 0; it < PTRS_MAX_ITERS; ++it) {
     const Words4 bw = key.block(c, (uint32_t)it);
-    *wptrs += 1;
+    d.iters += 1;
     const double U = u53(h ? bw.w2 : bw.w0, h ? bw.w3 : bw.w1) - 0.5;
     const double V = u53(h ? bw.w3 : bw.w1, h ? bw.w2 : bw.w0);
     const double us = 0.5 - __builtin_fabs(U);
     const double kf = __builtin_floor((2.0 * a / us + b) * U + lam + 0.43);
-    if (us >= 0.07 && V <= vr) return (int64_t)kf;
+    if (us >= 0.07 && V <= vr) {
+      d.k = (int64_t)kf;
+      break;
+    }
     if (kf < 0.0 || (us < 0.013 && V > us)) continue;
     const double lhs = log_f64(V) + log_f64(invalpha) - log_f64(a / (us * us) + b);
     const double rhs = -lam + kf * loglam - lgamma_int_f64(kf + 1.0);
-    if (lhs <= rhs) return (int64_t)kf;
+    if (lhs <= rhs) {
+      d.k = (int64_t)kf;
+      break;
+    }
   }
-  return -1;
+  return d;
 }
 
 constexpr float BIG_LAMBDA_F32 = 4194304.0f;  // 2^22
@@ -467,14 +477,12 @@ struct EngineF32 {
         const int j = __builtin_ctz(pend);
         pend &= pend - 1u;
         const float l = pick_m<MP>(lam, j);
-        unsigned int extra = 0;
-        const int64_t kk = (l >= BIG_LAMBDA_F32)
-                               ? ptrs_big_f64(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1,
-                                              (double)l, &extra)
-                               : ptrs_rest_f32(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1,
-                                               1, l, &extra);
-        cnt.w.ptrs += extra;
-        cnt.w.uniforms += 2 * extra;
+        const Draw dr = (l >= BIG_LAMBDA_F32)
+                            ? ptrs_big_f64(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1, (double)l)
+                            : ptrs_rest_f32(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1, 1, l);
+        cnt.w.ptrs += dr.iters;
+        cnt.w.uniforms += 2 * dr.iters;
+        const int64_t kk = dr.k;
         if (kk < 0) return FLAG_PTRS;
 #pragma unroll
         for (int r = 0; r < MP; ++r) K[r] = (r == j) ? (int32_t)kk : K[r];

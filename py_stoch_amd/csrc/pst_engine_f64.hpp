@@ -91,7 +91,7 @@ __device__ __forceinline__ bool ptrs_fast_f64(const PtrsF64& P, double U, double
 }
 
 // The rest of one PTRS iteration after a failed squeeze.
-__device__ __noinline__ bool ptrs_slow_f64(const PtrsF64& P, double V, double kf, double us) {
+__device__ __forceinline__ bool ptrs_slow_f64(const PtrsF64& P, double V, double kf, double us) {
   if (kf < 0.0 || (us < 0.013 && V > us)) return false;
   const double invalpha = 1.1239 + 1.1328 / (P.b - 3.4);
   const double lhs = log_f64(V) + log_f64(invalpha) - log_f64(P.a / (us * us) + P.b);
@@ -101,19 +101,27 @@ __device__ __noinline__ bool ptrs_slow_f64(const PtrsF64& P, double V, double kf
 
 // PTRS iterations it0, it0 + 1, ... of the draw whose iteration `it` uses
 // block (c, it); -1 at the iteration limit.  Used after iteration 0 failed.
-__device__ __noinline__ int64_t ptrs_rest_f64(const Key& key, uint64_t c, int it0, double lam, Work& w) {
+__device__ __noinline__ Draw ptrs_rest_f64(uint32_t k0, uint32_t k1, uint32_t s_lo, uint32_t s_hi, uint64_t c,
+                                           int it0, double lam) {
+  Key key;
+  key.k0 = k0;
+  key.k1 = k1;
+  key.s_lo = s_lo;
+  key.s_hi = s_hi;
   const PtrsF64 P = ptrs_setup_f64(lam);
+  Draw d = {-1, 0u};
   for (int it = it0; it < PTRS_MAX_ITERS; ++it) {
     const Words4 bw = key.block(c, (uint32_t)it);
-    w.ptrs += 1;
-    w.uniforms += 2;
+    d.iters += 1;
     const double U = u53(bw.w0, bw.w1) - 0.5;
     const double V = u53(bw.w2, bw.w3);
     double kf, us;
-    if (ptrs_fast_f64(P, U, V, kf, us)) return (int64_t)kf;
-    if (ptrs_slow_f64(P, V, kf, us)) return (int64_t)kf;
+    if (ptrs_fast_f64(P, U, V, kf, us) || ptrs_slow_f64(P, V, kf, us)) {
+      d.k = (int64_t)kf;
+      break;
+    }
   }
-  return -1;
+  return d;
 }
 
 // Per-lane state and steps of one FP64 simulation.
@@ -442,7 +450,10 @@ struct EngineF64 {
       while (pend != 0u) {
         const int j = __builtin_ctz(pend);
         pend &= pend - 1u;
-        const int64_t kk = ptrs_rest_f64(key, grp + (uint64_t)j, 1, pick<MP>(lam, j), cnt.w);
+        const Draw dr = ptrs_rest_f64(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)j, 1, pick<MP>(lam, j));
+        cnt.w.ptrs += dr.iters;
+        cnt.w.uniforms += 2 * dr.iters;
+        const int64_t kk = dr.k;
         if (kk < 0) return FLAG_PTRS;
 #pragma unroll
         for (int r = 0; r < MP; ++r)
