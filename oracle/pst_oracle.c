@@ -346,7 +346,7 @@ static double calc_g(const orc_network* net, int x_i, int i)
 static int32_t rn_f32_to_int(float f)
 {
     if (f != f) return 0;
-    if (f >= 2147483520.0f) return 2147483647;
+    if (f >= 2147483648.0f) return 2147483647;  /* 2^31 and above saturate */
     if (f <= -2147483648.0f) return (-2147483647 - 1);
     return (int32_t)nearbyintf(f);
 }

@@ -209,7 +209,7 @@ __device__ __forceinline__ double lgamma_int_f64(double x) {  // x = k + 1
 // __float2int_rn (P1_P2.py:572): round half to even; NaN -> 0, clamped.
 __device__ __forceinline__ int32_t rn_f32_to_int(float f) {
   if (f != f) return 0;
-  if (f >= 2147483520.0f) return 2147483647;
+  if (f >= 2147483648.0f) return 2147483647;  // 2^31 and above saturate
   if (f <= -2147483648.0f) return (-2147483647 - 1);
   return (int32_t)__builtin_rintf(f);
 }

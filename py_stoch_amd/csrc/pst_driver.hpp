@@ -52,7 +52,7 @@ __device__ __forceinline__ int32_t pick_i(const int32_t (&x)[NP], int idx) {
 //   int  start(job, sim)            -> initial mode, state loaded
 //   int  step<SSA_ONLY>(job, cnt, m) -> next mode (DONE may set flag)
 //   int  flag
-constexpr int SSA_LANES_MIN = 8;
+constexpr int SSA_LANES_MIN = 32;
 
 template <class Engine, class Ref>
 __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__ jobs, int n_jobs,

@@ -61,7 +61,7 @@ __device__ __forceinline__ T pick_m(const T (&x)[NP], int idx) {
 __device__ __forceinline__ float log1p_f32(float d) {
   const float u = 1.0f + d;
   const float m = u - 1.0f;
-  return (m == 0.0f) ? d : fast_log(u) * (d / m);
+  return (m == 0.0f) ? d : fast_log(u) * (d * fast_rcp(m));
 }
 
 struct PtrsF32 {
@@ -71,7 +71,7 @@ struct PtrsF32 {
 __device__ __forceinline__ PtrsF32 ptrs_setup_f32(float lam) {
   PtrsF32 P;
   P.lam = lam;
-  P.slam = __builtin_sqrtf(lam);
+  P.slam = __builtin_amdgcn_sqrtf(lam);  // hardware v_sqrt_f32 (1 ulp)
   P.b = 0.931f + 2.53f * P.slam;
   P.a = -0.059f + 0.02483f * P.b;
   P.vr = 0.9277f - 3.6224f * fast_rcp(P.b - 2.0f);
@@ -230,6 +230,18 @@ struct EngineF32 {
       fill_rest(tab, true, cnt);
       return MODE_DONE;
     }
+    // Philox blocks bc .. bc + NB, generated together as independent chains:
+    // block bc serves the one-block draw (SSA step, or tau2 and the critical
+    // reaction) and the first Poisson attempt takes NB consecutive blocks
+    // after it, or from bc when there is no one-block draw.
+    constexpr int NB = (MP + 1) / 2;
+    Words4 B[NB + 1];
+    B[0] = key.block(bc, 0);
+    if (!SSA_ONLY) {
+#pragma unroll
+      for (int r = 1; r <= NB; ++r)
+        if (2 * (r - 1) < M) B[r] = key.block(bc + (uint64_t)r, 0);
+    }
     float a[MP];
     propensities_f32<Ref>(tab, k, x, a);
     float a0 = 0.0f;
@@ -292,8 +304,9 @@ struct EngineF32 {
     }
     float tauX = __builtin_inff();
     uint32_t cw1 = 0;
-    if (asum > 0.0) {
-      const Words4 bw = key.block(bc, 0);
+    const bool one = asum > 0.0;
+    if (one) {
+      const Words4 bw = B[0];
       bc += 1;
       cnt.w.uniforms += ssa ? 2 : 1;
       cw1 = bw.w1;
@@ -337,7 +350,7 @@ struct EngineF32 {
       int32_t K[MP];
 #pragma unroll
       for (int j = 0; j < MP; ++j) K[j] = (sel == j) ? 1 : 0;
-      const int st = leap(tab, cnt, a, crit, tau, K, xn);
+      const int st = leap(tab, cnt, a, crit, tau, K, xn, B, one);
       if (st != 0) {
         fill_rest(tab, true, cnt);
         flag = st;
@@ -397,7 +410,8 @@ struct EngineF32 {
   // 2r+1 draw from the two halves of block (grp + r, it).  Returns 0 or a flag.
   __device__ __forceinline__ int leap(const Ref& tab, Counters& cnt,
                                       const float (&a)[MP], const bool (&crit)[MP], float tau,
-                                      int32_t (&K)[MP], int32_t (&xn)[NP]) {
+                                      int32_t (&K)[MP], int32_t (&xn)[NP], const Words4 (&B)[(MP + 1) / 2 + 1],
+                                      bool shifted) {
     constexpr int NB = (MP + 1) / 2;
     const auto& P = tab.p();
     const int M = P.M;
@@ -405,10 +419,21 @@ struct EngineF32 {
     for (int h = 0; h < MAX_HALVINGS; ++h) {
       const uint64_t grp = bc;
       bc += (uint64_t)nblk;
+      // first attempt: the blocks generated with the step's; retries: new ones
       Words4 W[NB];
 #pragma unroll
-      for (int r = 0; r < NB; ++r)
-        if (2 * r < M) W[r] = key.block(grp + (uint64_t)r, 0);
+      for (int r = 0; r < NB; ++r) {
+        if (2 * r < M) {
+          if (h == 0) {
+            W[r].w0 = shifted ? B[r + 1].w0 : B[r].w0;
+            W[r].w1 = shifted ? B[r + 1].w1 : B[r].w1;
+            W[r].w2 = shifted ? B[r + 1].w2 : B[r].w2;
+            W[r].w3 = shifted ? B[r + 1].w3 : B[r].w3;
+          } else {
+            W[r] = key.block(grp + (uint64_t)r, 0);
+          }
+        }
+      }
       float lam[MP];
       unsigned pend = 0, inv = 0, drawn = 0;
       bool bad = false;

@@ -65,6 +65,8 @@ def test_calculate_g_branches(oracle_mod, h, ht, x, expect):
     (10, 20, 0.0, 1.0, 1.0, 20),
     (50, 30, 2.0, 0.5, 2.25, 40),    # -10 exactly
     (50, 31, 2.0, 0.5, 2.125, 45),   # -4.75 -> -5
-    (7, 7, 1.0, 0.1, 1.05, 7)])
+    (7, 7, 1.0, 0.1, 1.05, 7),
+    (0, 2147483520, 0.0, 1.0, 1.0, 2147483520),  # largest float below 2^31 converts exactly
+    (0, 1 << 24, 0.0, 1.0, 1.0, 1 << 24)])
 def test_interpolation_rule(oracle_mod, x, xp, t_old, tau, rec, expect):
     assert oracle_mod.interpolate(x, xp, t_old, tau, rec) == expect
