@@ -158,9 +158,10 @@ std::string rtc_source(const HostNet& net, const JobParams& c, int precision) {
   arr(s, "irs", net.irs, N);
   arr(s, "E", E.data(), N);
   s += "};\n}  // namespace pst\n";
-  s += precision == PST_PRECISION_FP64 ? "#define PST_ENGINE EngineF64\n" : "#define PST_ENGINE EngineF32\n";
+  s += precision == PST_PRECISION_FP64 ? "#define PST_ENGINE EngineF64\n#define PST_KERNEL_NAME pst_net_fp64\n"
+                                       : "#define PST_ENGINE EngineF32\n#define PST_KERNEL_NAME pst_net_mixed\n";
   s += "extern \"C\" __global__ void __launch_bounds__(256)\n"
-       "pst_fixed_kernel(const pst::JobParams* __restrict__ jobs, int n_jobs, unsigned int* __restrict__ queue,\n"
+       "PST_KERNEL_NAME(const pst::JobParams* __restrict__ jobs, int n_jobs, unsigned int* __restrict__ queue,\n"
        "                 unsigned long long* __restrict__ stats) {\n"
        "  pst::run_driver<pst::PST_ENGINE<pst::FixedRef<pst::NetGen>>, pst::FixedRef<pst::NetGen>>(jobs, n_jobs, "
        "queue, stats);\n}\n";
@@ -215,7 +216,8 @@ int rtc_kernel(const std::string& source, int precision, RtcKernel* out, std::st
     return PST_EHIP;
   }
   RtcKernel k;
-  if ((e = hipModuleGetFunction(&k.fn, mod, "pst_fixed_kernel")) != hipSuccess) {
+  if ((e = hipModuleGetFunction(&k.fn, mod, precision == PST_PRECISION_FP64 ? "pst_net_fp64" : "pst_net_mixed")) !=
+      hipSuccess) {
     *err = std::string("hipModuleGetFunction: ") + hipGetErrorString(e);
     return PST_EHIP;
   }
