@@ -236,6 +236,119 @@ def cpu_baseline(tabs, rates, x0, I, E, budget_s=15.0):
                       "%.2f s, %s" % (n, U, dt, cpu_model)}
 
 
+def _time_batch(plan, O_list, K, W, stream):
+    import torch
+    ptrs = [O.data_ptr() for O in O_list]
+    strides = [O.shape[-1] for O in O_list]
+    for _ in range(W):
+        plan.execute(ptrs, strides, None, stream.cuda_stream)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(K):
+        plan.execute(ptrs, strides, None, stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return t1 - t0, e0.elapsed_time(e1) / K, plan.stats()
+
+
+def _ks_rows(A, B):
+    """Two-sample KS distance per row of [rows][n] int tensors (same n)."""
+    import torch
+    sa = torch.sort(A, dim=-1).values.contiguous()
+    sb = torch.sort(B, dim=-1).values.contiguous()
+    vals = torch.cat([sa, sb], dim=-1).contiguous()
+    n = A.shape[-1]
+    fa = torch.searchsorted(sa, vals, right=True).double() / n
+    fb = torch.searchsorted(sb, vals, right=True).double() / n
+    return (fa - fb).abs().max(dim=-1).values
+
+
+def other_workload(args):
+    """Throughput of the mixed path and the FP64 path (both with the network
+    compiled in) on one of the other configurations, with the accuracy
+    statistic that configuration is judged by."""
+    import torch
+    from py_stoch_amd import _native, workloads
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    wl = args.workload
+    if wl == "birth-death":
+        specs = workloads.birth_death()
+    elif wl in ("schlogl", "schlogl-ssa"):
+        specs = workloads.schlogl(method=2 if wl == "schlogl-ssa" else 0)
+    else:
+        specs = workloads.sweep()
+    O_list = [torch.empty((s.kappa, s.ita, s.n_sims), dtype=torch.int32, device=dev) for s in specs]
+    print("bench: %s, compiling kernels" % wl, file=sys.stderr, flush=True)
+    p_mix = _native.BatchPlan(workloads.jobs(specs, _native.PRECISION_MIXED))
+    p_f64 = _native.BatchPlan(workloads.jobs(specs, _native.PRECISION_FP64))
+    print("bench: %s, mixed" % wl, file=sys.stderr, flush=True)
+    wall_m, kms_m, st_m = _time_batch(p_mix, O_list, args.steps, args.warmup, stream)
+    O_mix = [O.clone() for O in O_list]
+    print("bench: %s, FP64" % wl, file=sys.stderr, flush=True)
+    k_f = max(1, args.steps // 2)
+    wall_f, kms_f, st_f = _time_batch(p_f64, O_list, k_f, 1, stream)
+    O_f64 = [O.clone() for O in O_list]
+    value = st_m["events"] * args.steps / wall_m
+    fp64_value = st_f["events"] * k_f / wall_f
+    acc = {}
+    if wl == "birth-death":
+        # exact law: X(t) ~ Poisson(100 (1 - e^{-t/10}))
+        t = torch.tensor(np.asarray(specs[0].kw["I"], np.float64), device=dev)
+        lam = 100.0 * (1.0 - torch.exp(-0.1 * t))
+        for name, O in (("mixed", O_mix[0]), ("fp64", O_f64[0])):
+            x = O[0].double()
+            m = x.mean(-1)
+            se = (lam / x.shape[-1]).sqrt().clamp_min(1e-12)
+            acc[name] = {"mean_max_abs_z_vs_exact": float(((m - lam).abs() / se)[1:].max().item())}
+    elif wl in ("schlogl", "schlogl-ssa"):
+        ks = _ks_rows(O_mix[0][0], O_f64[0][0])
+        # noise floor: FP64 with another block of simulation ids
+        p_f64.set_stream(specs[0].kw["seed"], specs[0].n_sims)
+        p_f64.execute([O_list[0].data_ptr()], [specs[0].n_sims], None, stream.cuda_stream)
+        torch.cuda.synchronize()
+        ks_floor = _ks_rows(O_list[0][0], O_f64[0][0])
+        x_m, x_f = O_mix[0][0, -1].double(), O_f64[0][0, -1].double()
+        acc = {"ks_max_X": float(ks[1:].max().item()), "ks_max_X_noise_floor": float(ks_floor[1:].max().item()),
+               "P(X(t_end)<300)": {"mixed": float((x_m < 300).double().mean().item()),
+                                     "fp64": float((x_f < 300).double().mean().item())},
+               "mean_X(t_end)": {"mixed": float(x_m.mean().item()), "fp64": float(x_f.mean().item())}}
+    else:
+        # per parameter set: final-time mean of each recorded species, mixed vs FP64,
+        # as z-scores (flagged simulations left out on both sides)
+        zs = []
+        for s, Om, Of in zip(specs, O_mix, O_f64):
+            per = s.kw["sims_per_set"]
+            n_sets = s.n_sims // per
+            for k in range(s.kappa):
+                a_ = Om[k, -1].double().reshape(n_sets, per)
+                b_ = Of[k, -1].double().reshape(n_sets, per)
+                se = ((a_.var(-1) + b_.var(-1)) / per).sqrt()
+                ok = se > 0
+                zs.append(((a_.mean(-1) - b_.mean(-1)).abs() / se.clamp_min(1e-12))[ok])
+        z = torch.cat(zs)
+        acc = {"sets_compared": int(z.numel()), "frac_abs_z_above_4": float((z > 4).double().mean().item()),
+               "median_abs_z": float(z.median().item()),
+               "flagged": {"mixed": int(st_m["flagged"]), "fp64": int(st_f["flagged"])}}
+    return {
+        "metric": "reaction events/s, 1 GPU (%s)" % wl, "value": value, "unit": "reaction events/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall_m / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": wl, "jobs": [{"name": s.name, "simulations": s.n_sims, "N": int(s.tables["N"]),
+                                             "M": int(s.tables["M"]), "samples": s.ita} for s in specs]},
+        "steps_per_s": (st_m["leaps"] + st_m["ssa_steps"]) * args.steps / wall_m,
+        "simulations_per_s": sum(sp.n_sims for sp in specs) * args.steps / wall_m,
+        "kernel_ms": kms_m, "fp64_specialized": {"value": fp64_value, "kernel_ms": kms_f},
+        "speedup_vs_fp64_specialized": value / fp64_value,
+        "counters": {k: v for k, v in st_m.items() if k != "kernel_ms"}, "accuracy_vs_fp64": acc,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,6 +357,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--acc-sets", type=int, default=16,
                     help="independent sets of 65536 simulations per path for the accuracy check")
+    ap.add_argument("--workload", default="lotka-volterra",
+                    choices=["lotka-volterra", "birth-death", "schlogl", "schlogl-ssa", "sweep"],
+                    help="lotka-volterra (default) is the headline measurement; the others measure "
+                         "the remaining configurations with the same JSON fields")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run one mixed-precision launch of the workload and one 256 MiB device copy "
                          "(for HBM byte counters under rocprofv3 --pmc), then exit")
@@ -253,6 +370,10 @@ def main():
               "their own range of simulation ids; multi-GPU runs are not part of this build.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "lotka-volterra":
+        print(json.dumps(other_workload(args)))
+        return
 
     import torch
     from py_stoch_amd import _native, models

@@ -15,8 +15,17 @@ from ...simulator import StochasticSimulator
 from .... import _native
 
 
+def _method(args):
+    return _native.METHOD_FORCED_SSA if int(getattr(args, "gillespie", 0)) else _native.METHOD_TAU_LEAPING
+
+
 class CuTauLeaping(StochasticSimulator):
     def run(self, my_args):
         self.calculate_sizes(my_args)
-        method = _native.METHOD_FORCED_SSA if int(getattr(my_args, "gillespie", 0)) else _native.METHOD_TAU_LEAPING
-        return self._run(my_args, method)
+        return self._run(my_args, _method(my_args))
+
+    def run_batch(self, args_list):
+        """Run a list of TlArgs (e.g. ``sim_maker.TLArgsList.make_list``) in
+        one batched launch; returns the list of ``[kappa][ita][U]`` arrays
+        (each ``U`` rounded up as by ``run``)."""
+        return self._run_batch(list(args_list), _method)

@@ -93,3 +93,20 @@ def test_mixed_specialised_matches_table_kernel(native):
             d = np.abs(np.searchsorted(x, grid, side="right") - np.searchsorted(y, grid, side="right")).max() / 4096
             assert d < crit, (k, f, d)
     assert abs(s_t["events"] / s_s["events"] - 1.0) < 0.02
+
+
+def test_simulator_run_batch_equals_single_runs(native):
+    """CuTauLeaping().run_batch over a parameter sweep (what sim_maker
+    produces) gives, set by set, the arrays of separate run() calls."""
+    from py_stoch_amd.mod.simulator.cuTauLeaping.cuTauLeaping import CuTauLeaping
+    args = []
+    for c2 in (0.005, 0.01, 0.02):
+        spn = models.lotka_volterra(c2=c2)
+        a = models.make_args(spn, E=[0, 1], t_max=2.0, ita=21, U=300, seed=8)
+        args.append(a)
+    sim = CuTauLeaping()
+    batch = sim.run_batch(args)
+    assert all(a.U == 320 for a in args)  # rounded up as run() does (block 64 below 640)
+    for a, O in zip(args, batch):
+        single = CuTauLeaping().run(a)
+        assert np.array_equal(O, single)
