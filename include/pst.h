@@ -27,7 +27,8 @@
  * reference's flattened char4 tables (parser.py:58-71) widened to int32 x 4.
  * The generated UpdatePropensities (parser.py:32-55) is replaced by a
  * mass-action table: a_j = rates[j] * prod_i prod_{q < prop_order[i][j]}
- * (x_i - prop_step[i][j] * q).
+ * (x_i - prop_step[i][j] * q), or, for other kinetic laws, by the hazard
+ * expressions of pst_network.hazards compiled into a network kernel.
  *
  * Threading: calls are synchronous unless stated; one plan may be executed by
  * one host thread at a time; separate plans are independent.
@@ -77,6 +78,16 @@ typedef struct pst_network {
   const int32_t* H_type;      /* N: TlArgs.H_type                            */
   const int32_t* prop_order;  /* N x M, species-major                        */
   const int32_t* prop_step;   /* N x M, species-major, 0 or 1                */
+  /* Kinetic laws that are not a mass-action product (the reference splices
+   * every hazard into its kernel as C, parser.py:32-55).  NULL: propensities
+   * from prop_order / prop_step and one rate per reaction per rates row.
+   * Otherwise M C expressions in X(i) (count of species i), C(k) (parameter
+   * k of the rates row) and R(v) (a number in the kernel's arithmetic type),
+   * compiled into the kernel (requires cfg.specialize = 1); each rates row
+   * then holds n_params parameters.                                          */
+  const char* const* hazards;
+  int32_t n_params;
+  int32_t reserved;           /* must be 0                                   */
 } pst_network;
 
 typedef struct pst_config {

@@ -42,6 +42,9 @@ class PstNetwork(ctypes.Structure):
         ("H_type", _P_I32),
         ("prop_order", _P_I32),
         ("prop_step", _P_I32),
+        ("hazards", ctypes.POINTER(ctypes.c_char_p)),
+        ("n_params", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -189,7 +192,9 @@ class _Marshalled:
         self.Ht = np.ascontiguousarray(tables["H_type"], np.int32).reshape(N)
         self.po = np.ascontiguousarray(tables["prop_order"], np.int32).reshape(N, M)
         self.ps = np.ascontiguousarray(tables["prop_step"], np.int32).reshape(N, M)
-        self.rates = np.ascontiguousarray(np.atleast_2d(np.asarray(rates, np.float64)).reshape(-1, M))
+        hz = tables.get("hazard_c")
+        n_rates = int(tables.get("n_params", 0)) if hz else M
+        self.rates = np.ascontiguousarray(np.atleast_2d(np.asarray(rates, np.float64)).reshape(-1, max(n_rates, 1)))
         self.x0 = np.ascontiguousarray(np.atleast_2d(np.asarray(x0, np.int32)).reshape(-1, N))
         self.I = np.ascontiguousarray(np.asarray(I, np.float32).reshape(-1))
         self.E = np.ascontiguousarray(np.asarray(E, np.int32).reshape(-1))
@@ -200,10 +205,16 @@ class _Marshalled:
             sims_per_set = max(1, int(n_sims)) if self.n_sets == 1 else None
         if sims_per_set is None:
             raise ValueError("sims_per_set is required with several parameter rows")
+        self.hazards = None
+        if hz:
+            self._hz_bytes = [h.encode() for h in hz]
+            self.hazards = (ctypes.c_char_p * len(hz))(*self._hz_bytes)
         self.net = PstNetwork(N, M, self.A.shape[0], _ptr(self.A, ctypes.c_int32), self.V.shape[0],
                               _ptr(self.V, ctypes.c_int32), _ptr(self.H, ctypes.c_int32),
                               _ptr(self.Ht, ctypes.c_int32), _ptr(self.po, ctypes.c_int32),
-                              _ptr(self.ps, ctypes.c_int32))
+                              _ptr(self.ps, ctypes.c_int32),
+                              ctypes.cast(self.hazards, ctypes.POINTER(ctypes.c_char_p)) if hz else None,
+                              n_rates if hz else 0, 0)
         self.cfg = PstConfig(int(method), int(precision), int(n_c), int(ssa_burst), float(eta),
                              float(t_max), self.I.shape[0], self.E.shape[0],
                              _ptr(self.I, ctypes.c_float), _ptr(self.E, ctypes.c_int32),

@@ -42,6 +42,7 @@ struct JobParams {
   int32_t n_sims;
   int32_t sims_per_set; // simulations sharing one rates/x0 row
   int32_t ssa_slice;    // SSA steps per pass while other lanes wait to decide (launch tuning)
+  int32_t n_rates;      // length of a rates row: M, or the parameter count of compiled hazards
   double eta;           // tl_parser.py:44
   double t_max;
   uint64_t seed;
@@ -83,6 +84,8 @@ struct JobDev {
 template <int NP, int MP>
 struct JobRef {
   static constexpr int kNP = NP, kMP = MP;
+  static constexpr int kKP = MP;          // rates per row: one per reaction
+  static constexpr bool kCustom = false;  // propensities from the mass-action table
   using Dev = JobDev<NP, MP>;
   static constexpr int NW = (NP * MP + 3) / 4;
   static constexpr int NS = (NP + 3) / 4;
@@ -140,6 +143,8 @@ struct JobRef {
 //   static constexpr int N, M, kappa, n_c, method, ssa_burst;
 //   static constexpr double eta;
 //   static constexpr int8_t v[N*M], ord[N*M], stp[N*M], H[N], Ht[N], irs[N], E[N];
+//   static constexpr bool custom; static constexpr int n_rates;
+//   template <class R> static void props(x, c, a)   (used when custom)
 // The job descriptor is the JobParams block alone.
 template <class Net>
 struct FixedParams : JobParams {
@@ -151,6 +156,12 @@ struct FixedParams : JobParams {
 template <class Net>
 struct FixedRef {
   static constexpr int kNP = Net::N, kMP = Net::M;
+  static constexpr int kKP = Net::n_rates;   // rates per row
+  static constexpr bool kCustom = Net::custom;  // compiled hazard expressions
+  template <class R>
+  __device__ __forceinline__ static void props(const int32_t (&x)[kNP], const R (&c)[kKP], R (&a)[kMP]) {
+    Net::template props<R>(x, c, a);
+  }
   using Dev = JobParams;
   FixedParams<Net> P;
   __device__ __forceinline__ void load(const JobParams* jp) { load_params_vgpr(&P, jp); }

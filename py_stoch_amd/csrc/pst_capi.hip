@@ -8,6 +8,7 @@
 // trajectories (and 16 counters) come back.
 #include <hip/hip_runtime.h>
 
+#include <cctype>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -40,6 +41,56 @@ int hip_fail(hipError_t e, const char* what) {
 constexpr int kMaxSpecies = 16;
 constexpr int kMaxReactions = 16;
 constexpr int kSsaSlice = 4;
+constexpr int kMaxParams = 256;
+
+// A compiled hazard may use X(i), C(k), R(v), numbers, + - * / ( ) , and these
+// functions; anything else (statements, preprocessor lines, other names) is
+// rejected before the expression reaches the compiler.
+bool hazard_ok(const char* h, std::string* why) {
+  static const char* const kNames[] = {"X", "C", "R", "exp", "log", "sqrt", "pow", "fabs", "fmin", "fmax",
+                                       "tanh", "exp2", "log2", "log10", "cbrt", "floor", "ceil"};
+  int depth = 0;
+  for (const char* p = h; *p;) {
+    const char ch = *p;
+    if (std::isalpha((unsigned char)ch) || ch == '_') {
+      const char* q = p;
+      while (std::isalnum((unsigned char)*q) || *q == '_') ++q;
+      const std::string name(p, q);
+      bool known = false;
+      for (const char* k : kNames) known = known || name == k;
+      if (!known) {
+        *why = "unknown name '" + name + "'";
+        return false;
+      }
+      p = q;
+    } else if (std::isdigit((unsigned char)ch) || ch == '.') {
+      // number, with an optional exponent
+      while (std::isdigit((unsigned char)*p) || *p == '.') ++p;
+      if (*p == 'e' || *p == 'E') {
+        ++p;
+        if (*p == '+' || *p == '-') ++p;
+        while (std::isdigit((unsigned char)*p)) ++p;
+      }
+    } else if (ch == '(' || ch == ')') {
+      depth += ch == '(' ? 1 : -1;
+      if (depth < 0) {
+        *why = "unbalanced parentheses";
+        return false;
+      }
+      ++p;
+    } else if (std::strchr("+-*/, ", ch)) {
+      ++p;
+    } else {
+      *why = std::string("character '") + ch + "' is not allowed";
+      return false;
+    }
+  }
+  if (depth != 0) {
+    *why = "unbalanced parentheses";
+    return false;
+  }
+  return true;
+}
 
 __global__ void zero_u32(unsigned int* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,6 +131,20 @@ int prepare(const pst_network* net, const pst_config* cfg, const double* rates, 
   if (net->v_size < 0 || (net->v_size > 0 && !net->V)) return fail(PST_EINVAL, "bad V table");
   if (!net->H || !net->H_type || !net->prop_order || !net->prop_step)
     return fail(PST_EINVAL, "null H, H_type, prop_order or prop_step");
+  if (net->reserved != 0) return fail(PST_EINVAL, "reserved network field must be 0");
+  const bool custom = net->hazards != nullptr;
+  if (custom) {
+    if (cfg->specialize != 1)
+      return fail(PST_EINVAL, "hazard expressions are compiled into a network kernel: set specialize = 1");
+    if (net->n_params < 0 || net->n_params > kMaxParams)
+      return fail(PST_EINVAL, "n_params=%d out of range [0, %d]", net->n_params, kMaxParams);
+    for (int j = 0; j < M; ++j) {
+      std::string why;
+      if (!net->hazards[j] || !net->hazards[j][0]) return fail(PST_EINVAL, "hazard %d is empty", j);
+      if (!hazard_ok(net->hazards[j], &why)) return fail(PST_EINVAL, "hazard %d: %s", j, why.c_str());
+    }
+  }
+  const int n_rates = custom ? net->n_params : M;
 
   std::memset(h, 0, sizeof(*h));
   h->N = N;
@@ -120,9 +185,10 @@ int prepare(const pst_network* net, const pst_config* cfg, const double* rates, 
     h->E[q] = (int8_t)s;
   }
   for (int64_t r = 0; r < need_sets; ++r) {
-    for (int j = 0; j < M; ++j) {
-      const double k = rates[r * M + j];
-      if (!(k >= 0.0) || !std::isfinite(k)) return fail(PST_EINVAL, "rate %d of row %lld must be finite and >= 0", j, (long long)r);
+    for (int j = 0; j < n_rates; ++j) {
+      const double k = rates[r * n_rates + j];
+      if (!std::isfinite(k)) return fail(PST_EINVAL, "rate %d of row %lld is not finite", j, (long long)r);
+      if (!custom && !(k >= 0.0)) return fail(PST_EINVAL, "rate %d of row %lld must be >= 0", j, (long long)r);
     }
     for (int i = 0; i < N; ++i)
       if (x0[r * N + i] < 0) return fail(PST_EINVAL, "initial count %d of row %lld is negative", i, (long long)r);
@@ -150,6 +216,7 @@ struct PlanJob {
   pst::JobParams p;  // O / o_stride / flags filled at execute time
   int cls = -1;      // size class of the table-driven kernel, or -1
   int fixed = -1;    // group of the specialised kernel, or -1
+  std::vector<std::string> hazards;  // compiled hazard expressions (empty: mass action)
   double* d_rates = nullptr;
   int32_t* d_x0 = nullptr;
   float* d_I = nullptr;
@@ -274,14 +341,20 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
     pj.net = nets[q];
     pj.cls = J.cfg.specialize ? -1 : pst::size_class(pj.net.N, pj.net.M, pj.net.kappa);
     const int N = J.net.n_species, M = J.net.n_reactions;
+    if (J.net.hazards)
+      for (int j = 0; j < M; ++j) pj.hazards.emplace_back(J.net.hazards[j]);
+    const int n_rates = J.net.hazards ? J.net.n_params : M;
     const int64_t sets = J.cfg.n_sims == 0 ? 1 : ((int64_t)J.cfg.n_sims + J.cfg.sims_per_set - 1) / J.cfg.sims_per_set;
-    if ((e = hipMalloc(&pj.d_rates, sizeof(double) * sets * M)) != hipSuccess ||
+    const int64_t rate_words = sets * (n_rates > 0 ? n_rates : 1);
+    if ((e = hipMalloc(&pj.d_rates, sizeof(double) * rate_words)) != hipSuccess ||
         (e = hipMalloc(&pj.d_x0, sizeof(int32_t) * sets * N)) != hipSuccess ||
         (e = hipMalloc(&pj.d_I, sizeof(float) * J.cfg.ita)) != hipSuccess) {
       pst_plan_destroy(p);
       return hip_fail(e, "hipMalloc(job tables)");
     }
-    if ((e = hipMemcpy(pj.d_rates, J.rates, sizeof(double) * sets * M, hipMemcpyHostToDevice)) != hipSuccess ||
+    if ((e = hipMemset(pj.d_rates, 0, sizeof(double) * rate_words)) != hipSuccess ||
+        (n_rates > 0 &&
+         (e = hipMemcpy(pj.d_rates, J.rates, sizeof(double) * sets * n_rates, hipMemcpyHostToDevice)) != hipSuccess) ||
         (e = hipMemcpy(pj.d_x0, J.x0, sizeof(int32_t) * sets * N, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(pj.d_I, J.cfg.I, sizeof(float) * J.cfg.ita, hipMemcpyHostToDevice)) != hipSuccess) {
       pst_plan_destroy(p);
@@ -299,6 +372,7 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
     P.n_sims = J.cfg.n_sims;
     P.sims_per_set = J.cfg.sims_per_set;
     P.ssa_slice = kSsaSlice;
+    P.n_rates = n_rates;
     P.eta = J.cfg.eta;
     P.t_max = J.cfg.t_max;
     P.seed = J.cfg.seed;
@@ -309,7 +383,7 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
     if (J.cfg.specialize) {
       pst::RtcKernel k;
       std::string err;
-      const int rc2 = pst::rtc_kernel(pst::rtc_source(pj.net, P, p->precision), p->precision, &k, &err);
+      const int rc2 = pst::rtc_kernel(pst::rtc_source(pj.net, P, p->precision, pj.hazards), p->precision, &k, &err);
       if (rc2 != PST_OK) {
         pst_plan_destroy(p);
         return fail(rc2, "job %d: %s", q, err.c_str());

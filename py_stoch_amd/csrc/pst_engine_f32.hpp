@@ -20,18 +20,22 @@
 namespace pst {
 
 template <class Ref, int NP = Ref::kNP, int MP = Ref::kMP>
-__device__ __forceinline__ void propensities_f32(const Ref& tab, const float (&k)[MP],
+__device__ __forceinline__ void propensities_f32(const Ref& tab, const float (&k)[Ref::kKP],
                                                  const int32_t (&x)[NP], float (&a)[MP]) {
+  if constexpr (Ref::kCustom) {
+    Ref::template props<float>(x, k, a);  // compiled hazard expressions
+  } else {
 #pragma unroll
-  for (int j = 0; j < MP; ++j) {
-    float v = k[j];
+    for (int j = 0; j < MP; ++j) {
+      float v = k[j];
 #pragma unroll
-    for (int i = 0; i < NP; ++i) {
-      const int p = tab.Ord(i, j);
-      const int st = tab.Stp(i, j);
-      for (int q = 0; q < p; ++q) v = v * (float)(x[i] - st * q);
+      for (int i = 0; i < NP; ++i) {
+        const int p = tab.Ord(i, j);
+        const int st = tab.Stp(i, j);
+        for (int q = 0; q < p; ++q) v = v * (float)(x[i] - st * q);
+      }
+      a[j] = v;
     }
-    a[j] = v;
   }
 }
 
@@ -177,7 +181,7 @@ constexpr float BIG_LAMBDA_F32 = 4194304.0f;  // 2^22
 template <class Ref>
 struct EngineF32 {
   static constexpr int NP = Ref::kNP, MP = Ref::kMP;
-  float k[MP];
+  float k[Ref::kKP];  // rates row: per-reaction constants, or hazard parameters
   int32_t x[NP];
   Key key;
   uint64_t bc;
@@ -203,7 +207,7 @@ struct EngineF32 {
     const auto& P = jr.p();
     const int64_t set = sim / P.sims_per_set;
 #pragma unroll
-    for (int j = 0; j < MP; ++j) k[j] = (j < P.M) ? (float)P.rates[set * P.M + j] : 0.0f;
+    for (int j = 0; j < Ref::kKP; ++j) k[j] = (j < P.n_rates) ? (float)P.rates[set * P.n_rates + j] : 0.0f;
 #pragma unroll
     for (int i = 0; i < NP; ++i) x[i] = (i < P.N) ? P.x0[set * P.N + i] : 0;
     key.init(P.seed, P.sim_offset + (uint64_t)sim);

@@ -26,19 +26,23 @@
 namespace pst {
 
 template <class Ref, int NP = Ref::kNP, int MP = Ref::kMP>
-__device__ __forceinline__ void propensities_f64(const Ref& tab, const double (&k)[MP],
+__device__ __forceinline__ void propensities_f64(const Ref& tab, const double (&k)[Ref::kKP],
                                                  const int32_t (&x)[NP], double (&a)[MP]) {
-  // UpdatePropensities (parser.py:32-55) in table form, species ascending
+  if constexpr (Ref::kCustom) {
+    Ref::template props<double>(x, k, a);  // compiled hazard expressions
+  } else {
+    // UpdatePropensities (parser.py:32-55) in table form, species ascending
 #pragma unroll
-  for (int j = 0; j < MP; ++j) {
-    double v = k[j];
+    for (int j = 0; j < MP; ++j) {
+      double v = k[j];
 #pragma unroll
-    for (int i = 0; i < NP; ++i) {
-      const int p = tab.Ord(i, j);
-      const int st = tab.Stp(i, j);
-      for (int q = 0; q < p; ++q) v = v * (double)(x[i] - st * q);
+      for (int i = 0; i < NP; ++i) {
+        const int p = tab.Ord(i, j);
+        const int st = tab.Stp(i, j);
+        for (int q = 0; q < p; ++q) v = v * (double)(x[i] - st * q);
+      }
+      a[j] = v;
     }
-    a[j] = v;
   }
 }
 
@@ -128,7 +132,7 @@ __device__ __noinline__ Draw ptrs_rest_f64(uint32_t k0, uint32_t k1, uint32_t s_
 template <class Ref>
 struct EngineF64 {
   static constexpr int NP = Ref::kNP, MP = Ref::kMP;
-  double k[MP];
+  double k[Ref::kKP];  // rates row: per-reaction constants, or hazard parameters
   int32_t x[NP];
   Key key;
   uint64_t bc;  // Philox block counter
@@ -154,7 +158,7 @@ struct EngineF64 {
     const auto& P = jr.p();
     const int64_t set = sim / P.sims_per_set;
 #pragma unroll
-    for (int j = 0; j < MP; ++j) k[j] = (j < P.M) ? P.rates[set * P.M + j] : 0.0;
+    for (int j = 0; j < Ref::kKP; ++j) k[j] = (j < P.n_rates) ? P.rates[set * P.n_rates + j] : 0.0;
 #pragma unroll
     for (int i = 0; i < NP; ++i) x[i] = (i < P.N) ? P.x0[set * P.N + i] : 0;
     key.init(P.seed, P.sim_offset + (uint64_t)sim);

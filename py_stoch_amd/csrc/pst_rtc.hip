@@ -132,7 +132,8 @@ int compile(const std::string& source, int precision, std::vector<char>* code, s
 
 }  // namespace
 
-std::string rtc_source(const HostNet& net, const JobParams& c, int precision) {
+std::string rtc_source(const HostNet& net, const JobParams& c, int precision,
+                       const std::vector<std::string>& hazards) {
   const int N = net.N, M = net.M;
   std::vector<int8_t> v(N * M), ord(N * M), stp(N * M), E(N, 0);
   for (int i = 0; i < N * M; ++i) {
@@ -157,7 +158,18 @@ std::string rtc_source(const HostNet& net, const JobParams& c, int precision) {
   arr(s, "Ht", net.Ht, N);
   arr(s, "irs", net.irs, N);
   arr(s, "E", E.data(), N);
-  s += "};\n}  // namespace pst\n";
+  const bool custom = !hazards.empty();
+  const int n_rates = custom ? c.n_rates : M;
+  s += std::string("  static constexpr bool custom = ") + (custom ? "true" : "false") + ";\n";
+  s += "  static constexpr int n_rates = " + std::to_string(n_rates > 0 ? n_rates : 1) + ";\n";
+  s += "  template <class R>\n  __device__ __forceinline__ static void props(const int32_t (&xs)[N], const R (&cs)[n_rates],"
+       " R (&a)[M]) {\n";
+  if (custom) {
+    s += "#define X(i) ((R)xs[i])\n#define C(k) (cs[k])\n";
+    for (int j = 0; j < M; ++j) s += "    a[" + std::to_string(j) + "] = " + hazards[j] + ";\n";
+    s += "#undef X\n#undef C\n";
+  }
+  s += "  }\n};\n}  // namespace pst\n";
   s += precision == PST_PRECISION_FP64 ? "#define PST_ENGINE EngineF64\n#define PST_KERNEL_NAME pst_net_fp64\n"
                                        : "#define PST_ENGINE EngineF32\n#define PST_KERNEL_NAME pst_net_mixed\n";
   s += "extern \"C\" __global__ void __launch_bounds__(256)\n"

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "pst_args.hpp"
 
@@ -29,8 +30,11 @@ struct RtcKernel {
 };
 
 // Source text of the specialised kernel for a network and the constant
-// settings in `c` (N, M, kappa, n_c, eta, method, ssa_burst are read).
-std::string rtc_source(const HostNet& net, const JobParams& c, int precision);
+// settings in `c` (N, M, kappa, n_c, eta, method, ssa_burst, n_rates are
+// read).  `hazards` empty: mass-action propensities from the table; else one
+// C expression per reaction (X(i), C(k), R(v), see pst_network.hazards).
+std::string rtc_source(const HostNet& net, const JobParams& c, int precision,
+                       const std::vector<std::string>& hazards);
 
 // Compile (or fetch from the cache) the kernel for `source` on the current
 // device.  Returns 0, or a PST_* status with a message in err.

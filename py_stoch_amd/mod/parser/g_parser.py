@@ -41,10 +41,13 @@ class GParser(Parser):
         args_out.N = len(spn.P)
         args_out.hazards = GParser.define_hazards(spn)
         n_params = len(np.atleast_1d(np.asarray(spn.c)))
-        order, step, coeffs = mass_action.analyse(spn.h, args_out.N, n_params)
+        # mass-action table, or C expressions compiled into the kernel
+        order, step, rate_fn, hazard_c = mass_action.propensity_tables(spn.h, args_out.N, n_params)
         args_out.prop_order = order
         args_out.prop_step = step
-        args_out.rate_fn = mass_action.RateFunction(coeffs, n_params)
+        args_out.rate_fn = rate_fn
+        args_out.hazard_c = hazard_c
+        args_out.n_params = n_params
         args_out.kappa = len(args_out.E)
         args_out.I = np.linspace(0, args_out.t_max, num=args_out.ita).astype(np.float32)
         return args_out

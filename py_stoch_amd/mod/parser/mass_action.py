@@ -95,3 +95,85 @@ class RateFunction:
             raise ValueError("%d parameter values given, %d needed" % (c.shape[0], self.n_params))
         vals = [float(v) for v in c[: self.n_params]]
         return np.array([float(f(*vals)) for f in self._fns], np.float64)
+
+
+# ---------------------------------------------------------------------------
+# hazards that are not a mass-action product: compiled as C expressions
+# ---------------------------------------------------------------------------
+from sympy.printing.c import C99CodePrinter  # noqa: E402
+
+
+class _HazardPrinter(C99CodePrinter):
+    """C for one hazard, in the kernel's arithmetic type R: species as X(i)
+    (the count converted to R), parameters as C(k), every number as R(...),
+    small integer powers as products (the reference splices the hazard
+    string into its kernel instead, parser.py:32-55)."""
+
+    def _print_Symbol(self, expr):
+        name = expr.name
+        if name.startswith("species_"):
+            return "X(%d)" % int(name[8:])
+        if name.startswith("parameter_"):
+            return "C(%d)" % int(name[10:])
+        raise UnsupportedHazard("unknown symbol %s in a hazard" % name)
+
+    def _print_Integer(self, expr):
+        return "R(%d.0)" % int(expr)
+
+    def _print_Rational(self, expr):
+        return "(R(%d.0) / R(%d.0))" % (int(expr.p), int(expr.q))
+
+    def _print_Float(self, expr):
+        return "R(%s)" % repr(float(expr))
+
+    def _print_Pow(self, expr):
+        b, e = expr.as_base_exp()
+        if e.is_Integer and 1 <= int(e) <= 6:
+            return "(" + "*".join([self.parenthesize(b, 100)] * int(e)) + ")"
+        if e.is_Integer and -6 <= int(e) <= -1:
+            return "(R(1.0) / (" + "*".join([self.parenthesize(b, 100)] * (-int(e))) + "))"
+        if e == sympy.Rational(1, 2):
+            return "sqrt(%s)" % self._print(b)
+        return "pow(%s, %s)" % (self._print(b), self._print(e))
+
+
+def hazard_code(hazards, n_species, n_params):
+    """One C expression per hazard (see _HazardPrinter)."""
+    S = [sympy.Symbol("species_%d" % i) for i in range(n_species)]
+    Pm = [sympy.Symbol("parameter_%d" % k) for k in range(n_params)]
+    names = {str(s): s for s in S + Pm}
+    pr = _HazardPrinter()
+    out = []
+    for j, h in enumerate(hazards):
+        expr = sympy.sympify(h.replace("^", "**"), locals=names)
+        bad = [s for s in expr.free_symbols if s not in S and s not in Pm]
+        if bad:
+            raise UnsupportedHazard("hazard %d (%s) uses unknown symbols %s" % (j, h, bad))
+        out.append(pr.doprint(expr))
+    return out
+
+
+class ParameterRows:
+    """Rates function of a network with compiled hazards: the kernel reads
+    the parameter vector itself."""
+
+    def __init__(self, n_params):
+        self.n_params = n_params
+
+    def __call__(self, c):
+        c = np.atleast_1d(np.asarray(c, np.float64)).reshape(-1)
+        if c.shape[0] < self.n_params:
+            raise ValueError("%d parameter values given, %d needed" % (c.shape[0], self.n_params))
+        return c[: self.n_params].copy() if self.n_params else np.zeros(1, np.float64)
+
+
+def propensity_tables(hazards, n_species, n_params):
+    """(prop_order, prop_step, rate_fn, hazard_c): the mass-action table when
+    every hazard has that form (hazard_c None), else the C expressions."""
+    try:
+        order, step, coeffs = analyse(hazards, n_species, n_params)
+        return order, step, RateFunction(coeffs, n_params), None
+    except UnsupportedHazard:
+        M = len(hazards)
+        return (np.zeros((n_species, M), np.int32), np.ones((n_species, M), np.int32), ParameterRows(n_params),
+                hazard_code(hazards, n_species, n_params))

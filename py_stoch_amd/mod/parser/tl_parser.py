@@ -80,10 +80,13 @@ class TlParser(Parser):
         args_out.hazards = TlParser.define_hazards(spn)
 
         n_params = len(np.atleast_1d(np.asarray(spn.c)))
-        order, step, coeffs = mass_action.analyse(spn.h, args_out.N, n_params)
+        # mass-action table, or C expressions compiled into the kernel
+        order, step, rate_fn, hazard_c = mass_action.propensity_tables(spn.h, args_out.N, n_params)
         args_out.prop_order = order
         args_out.prop_step = step
-        args_out.rate_fn = mass_action.RateFunction(coeffs, n_params)
+        args_out.rate_fn = rate_fn
+        args_out.hazard_c = hazard_c
+        args_out.n_params = n_params
 
         args_out.kappa = len(args_out.E)
         args_out.I = np.linspace(0, args_out.t_max, num=args_out.ita).astype(np.float32)
@@ -131,4 +134,6 @@ def tables_of(args):
     Ht = np.asarray(getattr(args, "H_type", np.zeros(N)), np.int32).reshape(N) if len(getattr(args, "H_type", [])) else np.zeros(N, np.int32)
     return {"N": N, "M": M, "A": A, "V": V, "H": H, "H_type": Ht,
             "prop_order": np.asarray(args.prop_order, np.int32).reshape(N, M),
-            "prop_step": np.asarray(args.prop_step, np.int32).reshape(N, M)}
+            "prop_step": np.asarray(args.prop_step, np.int32).reshape(N, M),
+            "hazard_c": getattr(args, "hazard_c", None),
+            "n_params": int(getattr(args, "n_params", 0))}

@@ -66,6 +66,20 @@ def schlogl(c1=3e-7, c2=1e-4, c3=1e-3, c4=3.5, x0=250, a=100000, b=200000):
                 [x0, a, b], [c1, c2, c3, c4], constant=(A, B))
 
 
+def michaelis_menten_birth_death(k=20.0, vmax=25.0, km=10.0, x0=0):
+    """0 -> X at rate k, X -> 0 at rate vmax X / (km + X): a kinetic law that
+    is not a mass-action product (compiled into the kernel)."""
+    return _spn(1, [({}, {0: 1}, "parameter_0"),
+                    ({0: 1}, {}, "parameter_1*species_0/(parameter_2 + species_0)")], [x0], [k, vmax, km])
+
+
+def immigration_birth_death(nu=5.0, b=0.5, d=1.0, x0=0):
+    """0 -> X with hazard nu + b X (immigration plus births, one polynomial
+    that is not a single product), X -> 0 at rate d X."""
+    return _spn(1, [({}, {0: 1}, "parameter_0 + parameter_1*species_0"),
+                    ({0: 1}, {}, "parameter_2*species_0")], [x0], [nu, b, d])
+
+
 def random_network(rng, n_species=None, n_reactions=None, max_species=50, max_reactions=200):
     """Random mass-action network: reactant order 0-2 (species drawn with
     replacement), 0-3 product species with stoichiometry 1-2, rates

@@ -110,3 +110,54 @@ def test_simulator_run_batch_equals_single_runs(native):
     for a, O in zip(args, batch):
         single = CuTauLeaping().run(a)
         assert np.array_equal(O, single)
+
+
+def _mm_stationary_cdf(k, vmax, km, n_max=400):
+    """Birth-death chain 0 -> X (k), X -> 0 (vmax X / (km + X)): detailed
+    balance gives pi(n) / pi(n-1) = k / (vmax n / (km + n))."""
+    logp = np.zeros(n_max + 1)
+    for n in range(1, n_max + 1):
+        logp[n] = logp[n - 1] + np.log(k) - np.log(vmax * n / (km + n))
+    p = np.exp(logp - logp.max())
+    return np.cumsum(p / p.sum())
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_michaelis_menten_stationary_law(native, precision):
+    """Pure SSA with a compiled non-mass-action kinetic law reaches the exact
+    stationary law: one-sample KS distance at t = 100 below 0.025 (the 0.01%
+    critical value is 0.0246 for 8192 draws of a continuous law; discreteness
+    only lowers the statistic).  The chain relaxes at rate d/dn[vmax n /
+    (km + n)] = 0.1 at its mean n = 40, where it starts, so e^-10 of the
+    start remains."""
+    from py_stoch_amd.mod.parser.g_parser import GArgs, GParser
+    a = GArgs()
+    a.E, a.t_max, a.ita, a.U = [0], 100.0, 11, 8192
+    a = GParser.parse(a, models.michaelis_menten_birth_death(k=20.0, vmax=25.0, km=10.0, x0=40))
+    print("case: michaelis-menten precision=%d" % precision, flush=True)
+    O, f, st = native.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, method=native.METHOD_PURE_SSA,
+                               precision=precision, t_max=100.0, I=a.I, E=a.E, n_sims=8192, seed=12,
+                               specialize=True)
+    assert (f == 0).all()
+    x = O[0, -1]
+    cdf = _mm_stationary_cdf(20.0, 25.0, 10.0)
+    emp = np.searchsorted(np.sort(x), np.arange(cdf.shape[0]), side="right") / x.shape[0]
+    assert np.abs(emp - cdf).max() < 0.025
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_immigration_birth_death_mean(native, precision):
+    """Tau-leaping with a compiled polynomial hazard nu + b X: the mean follows
+    m(t) = nu / (d - b) (1 - e^{-(d - b) t}) (linear propensities), within
+    4 standard errors plus 2% for the leap bias."""
+    a = models.make_args(models.immigration_birth_death(nu=50.0, b=0.5, d=1.0), E=[0], t_max=10.0, ita=11, U=8192)
+    print("case: immigration-birth-death precision=%d" % precision, flush=True)
+    O, f, st = native.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, method=native.METHOD_TAU_LEAPING,
+                               precision=precision, t_max=10.0, I=a.I, E=a.E, n_sims=8192, seed=13,
+                               specialize=True)
+    assert (f == 0).all() and st["leaps"] > 0
+    t = np.asarray(a.I, np.float64)
+    m = 50.0 / 0.5 * (1.0 - np.exp(-0.5 * t))
+    mean = O[0].mean(axis=-1)
+    se = np.sqrt(np.maximum(O[0].var(axis=-1), 1.0) / O.shape[-1])
+    assert np.all(np.abs(mean - m) <= 4 * se + 0.02 * m)
