@@ -222,6 +222,14 @@ int launch_mixed(int cls, const void* jobs, int n_jobs, unsigned int* queue, uns
 int blocks_per_cu_fp64(int cls);
 int blocks_per_cu_mixed(int cls);
 
+// the large-network kernel (one wavefront per simulation, mixed precision):
+// `jobs` is a device array of WaveJob
+struct WaveTab;
+int launch_wave(const void* jobs, int n_jobs, unsigned int* queue, unsigned long long* stats, int waves,
+                hipStream_t stream);
+int blocks_per_cu_wave();
+bool make_wave_tab(const HostNet& h, WaveTab* t, char* why, size_t why_len);
+
 // smallest compiled size class holding (N, M, kappa), or -1
 inline int size_class(int N, int M, int kappa) {
   for (int c = 0; c < NUM_CLASSES; ++c)

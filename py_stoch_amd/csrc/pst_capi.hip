@@ -20,6 +20,7 @@
 
 #include "../../include/pst.h"
 #include "pst_args.hpp"
+#include "pst_engine_wave.hpp"
 #include "pst_rtc.hpp"
 
 namespace {
@@ -38,8 +39,9 @@ int hip_fail(hipError_t e, const char* what) {
   return fail(e == hipErrorOutOfMemory ? PST_ENOMEM : PST_EHIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-constexpr int kMaxSpecies = 16;
-constexpr int kMaxReactions = 16;
+// largest network of any kernel (the large-network kernel, mixed precision)
+constexpr int kMaxSpecies = pst::WAVE_MAX_N;
+constexpr int kMaxReactions = pst::WAVE_MAX_M;
 constexpr int kSsaSlice = 4;
 constexpr int kMaxParams = 256;
 
@@ -216,6 +218,8 @@ struct PlanJob {
   pst::JobParams p;  // O / o_stride / flags filled at execute time
   int cls = -1;      // size class of the table-driven kernel, or -1
   int fixed = -1;    // group of the specialised kernel, or -1
+  bool wave = false; // large-network kernel (one wavefront per simulation)
+  std::vector<unsigned char> wave_tab;  // its sparse table (a WaveTab)
   std::vector<std::string> hazards;  // compiled hazard expressions (empty: mass action)
   double* d_rates = nullptr;
   int32_t* d_x0 = nullptr;
@@ -240,6 +244,32 @@ void pack_jobs(const std::vector<PlanJob>& jobs, int cls, std::vector<unsigned c
 
 }  // namespace
 
+enum { KIND_TABLE = 0, KIND_FIXED = 1, KIND_WAVE = 2 };
+
+// Which kernel runs a job: a network-specific kernel (specialize = 1, or
+// compiled hazards), a table-driven size class, or -- networks beyond both,
+// mixed precision -- the large-network kernel.  Negative: error status.
+int kernel_kind(const pst_job& J, const pst::HostNet& h) {
+  const bool fits_rtc = h.N <= pst::RTC_MAX_SPECIES && h.M <= pst::RTC_MAX_REACTIONS;
+  if (J.net.hazards) {
+    if (!fits_rtc)
+      return fail(PST_ETOOLARGE, "compiled hazard expressions: network kernels take N<=%d, M<=%d (N=%d, M=%d)",
+                  pst::RTC_MAX_SPECIES, pst::RTC_MAX_REACTIONS, h.N, h.M);
+    return KIND_FIXED;
+  }
+  if (J.cfg.specialize && fits_rtc) return KIND_FIXED;
+  if (pst::size_class(h.N, h.M, h.kappa) >= 0) return KIND_TABLE;
+  if (J.cfg.precision != PST_PRECISION_MIXED)
+    return fail(PST_ETOOLARGE,
+                "N=%d, M=%d: FP64 kernels take N<=16 and M<=16 (M<=%d with specialize=1); larger networks run in "
+                "mixed precision", h.N, h.M, pst::RTC_MAX_REACTIONS);
+  std::vector<unsigned char> buf(sizeof(pst::WaveTab));
+  char why[160];
+  if (!pst::make_wave_tab(h, (pst::WaveTab*)buf.data(), why, sizeof(why)))
+    return fail(PST_ETOOLARGE, "large-network kernel: %s", why);
+  return KIND_WAVE;
+}
+
 // jobs sharing one specialised kernel (same network and constant settings)
 struct FixedGroup {
   pst::RtcKernel k;
@@ -258,6 +288,9 @@ struct pst_plan {
   size_t d_jobs_bytes[pst::NUM_CLASSES] = {0, 0, 0};
   std::vector<unsigned char> uploaded[pst::NUM_CLASSES];  // last descriptors copied to d_jobs
   std::vector<FixedGroup> fixed;
+  void* d_wave = nullptr;  // WaveJob array of the large-network jobs
+  size_t d_wave_bytes = 0;
+  std::vector<unsigned char> uploaded_wave;
   unsigned int* d_queue = nullptr;
   unsigned long long* d_stats = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -286,6 +319,7 @@ void pst_plan_destroy(pst_plan* p) {
     if (p->d_jobs[c]) (void)hipFree(p->d_jobs[c]);
   for (FixedGroup& g : p->fixed)
     if (g.d_jobs) (void)hipFree(g.d_jobs);
+  if (p->d_wave) (void)hipFree(p->d_wave);
   if (p->d_queue) (void)hipFree(p->d_queue);
   if (p->d_stats) (void)hipFree(p->d_stats);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
@@ -300,6 +334,7 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
   if (!jobs || n_jobs < 1) return fail(PST_EINVAL, "need at least one job");
   // validate every job on the host before touching the device
   std::vector<pst::HostNet> nets(n_jobs);
+  std::vector<int> kinds(n_jobs, 0);
   for (int q = 0; q < n_jobs; ++q) {
     const pst_job& J = jobs[q];
     int rc = prepare(&J.net, &J.cfg, J.rates, J.x0, J.n_sets, &nets[q]);
@@ -316,14 +351,9 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
                   J.cfg.precision, jobs[0].cfg.precision);
     if (J.cfg.specialize != 0 && J.cfg.specialize != 1) return fail(PST_EINVAL, "job %d: specialize must be 0 or 1", q);
     if (J.cfg.reserved != 0) return fail(PST_EINVAL, "job %d: reserved config field must be 0", q);
-    if (J.cfg.specialize) {
-      if (nets[q].N > pst::RTC_MAX_SPECIES || nets[q].M > pst::RTC_MAX_REACTIONS)
-        return fail(PST_ETOOLARGE, "job %d: N=%d, M=%d is too large for a specialised kernel (N<=%d, M<=%d)", q,
-                    nets[q].N, nets[q].M, pst::RTC_MAX_SPECIES, pst::RTC_MAX_REACTIONS);
-    } else if (pst::size_class(nets[q].N, nets[q].M, nets[q].kappa) < 0) {
-      return fail(PST_ETOOLARGE, "job %d: no compiled kernel for N=%d, M=%d, kappa=%d", q, nets[q].N, nets[q].M,
-                  nets[q].kappa);
-    }
+    const int kd = kernel_kind(J, nets[q]);
+    if (kd < 0) return kd;
+    kinds[q] = kd;
   }
   int cu = 0;
   int rc = check_device(&cu);
@@ -339,7 +369,13 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
     const pst_job& J = jobs[q];
     PlanJob& pj = p->jobs[q];
     pj.net = nets[q];
-    pj.cls = J.cfg.specialize ? -1 : pst::size_class(pj.net.N, pj.net.M, pj.net.kappa);
+    pj.cls = kinds[q] == KIND_TABLE ? pst::size_class(pj.net.N, pj.net.M, pj.net.kappa) : -1;
+    if (kinds[q] == KIND_WAVE) {
+      pj.wave = true;
+      pj.wave_tab.resize(sizeof(pst::WaveTab));
+      char why[160];
+      (void)pst::make_wave_tab(pj.net, (pst::WaveTab*)pj.wave_tab.data(), why, sizeof(why));
+    }
     const int N = J.net.n_species, M = J.net.n_reactions;
     if (J.net.hazards)
       for (int j = 0; j < M; ++j) pj.hazards.emplace_back(J.net.hazards[j]);
@@ -380,7 +416,7 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
     P.rates = pj.d_rates;
     P.x0 = pj.d_x0;
     P.I = pj.d_I;
-    if (J.cfg.specialize) {
+    if (kinds[q] == KIND_FIXED) {
       pst::RtcKernel k;
       std::string err;
       const int rc2 = pst::rtc_kernel(pst::rtc_source(pj.net, P, p->precision, pj.hazards), p->precision, &k, &err);
@@ -485,6 +521,35 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
       return hip_fail(e, "hipMemcpy(jobs)");
     g.uploaded = bytes;
   }
+  // large-network jobs: parameters and sparse table per job
+  int n_wave = 0;
+  int64_t wave_sims = 0;
+  {
+    std::vector<unsigned char> bytes;
+    for (const PlanJob& pj : p->jobs) {
+      if (!pj.wave) continue;
+      const size_t off = bytes.size();
+      bytes.resize(off + sizeof(pst::WaveJob));
+      pst::WaveJob* wj = (pst::WaveJob*)(bytes.data() + off);
+      std::memcpy(&wj->p, &pj.p, sizeof(pst::JobParams));
+      std::memcpy(&wj->t, pj.wave_tab.data(), sizeof(pst::WaveTab));
+      ++n_wave;
+      wave_sims += pj.p.n_sims;
+    }
+    if (n_wave > 0 && bytes != p->uploaded_wave) {
+      if (p->d_wave_bytes < bytes.size()) {
+        if (p->d_wave) (void)hipFree(p->d_wave);
+        p->d_wave = nullptr;
+        p->d_wave_bytes = 0;
+        if ((e = hipMalloc(&p->d_wave, bytes.size())) != hipSuccess) return hip_fail(e, "hipMalloc(jobs)");
+        p->d_wave_bytes = bytes.size();
+      }
+      if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+      if ((e = hipMemcpy(p->d_wave, bytes.data(), bytes.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy(jobs)");
+      p->uploaded_wave = bytes;
+    }
+  }
   hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, s, p->d_stats, (int)pst::ST_COUNT);
   hipLaunchKernelGGL(zero_u32, dim3((n_jobs + 255) / 256), dim3(256), 0, s, p->d_queue, n_jobs);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "zero kernels");
@@ -516,6 +581,14 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
         hipSuccess)
       return hip_fail(e, "specialised kernel launch");
     qoff += cnt;
+  }
+  if (n_wave > 0) {
+    // one simulation per wavefront at a time
+    const int64_t resident = (int64_t)pst::blocks_per_cu_wave() * p->cu_count * 4;
+    const int waves = (int)(wave_sims < resident ? wave_sims : resident);
+    const int rc = pst::launch_wave(p->d_wave, n_wave, p->d_queue + qoff, p->d_stats, waves, s);
+    if (rc != 0) return hip_fail((hipError_t)rc, "large-network kernel launch");
+    qoff += n_wave;
   }
   if ((e = hipEventRecord(p->ev1, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
   p->executed = true;

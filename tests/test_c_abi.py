@@ -31,7 +31,22 @@ def test_library_exports_every_declared_function(native):
 
 def test_abi_version_and_limits(native):
     assert native.load().pst_abi_version() == 2
-    assert native.max_network() == (16, 16)
+    assert native.max_network() == (64, 256)
+
+
+def test_large_networks_need_mixed_precision(native):
+    """Beyond the FP64 size classes (16 x 16, 16 x 32 compiled) networks run
+    on the large-network kernel, which is mixed precision only; beyond 64
+    species or 256 reactions nothing runs.  Both are refused on the host."""
+    rng = np.random.default_rng(1)
+    a = models.make_args(models.random_network(rng, n_species=20, n_reactions=30), E=[0], t_max=1.0, ita=3, U=64)
+    with pytest.raises(native.NativeError, match="larger networks run in mixed precision"):
+        native.Plan(tables_of(a), a.rate_fn(a.c), a.x_0, precision=native.PRECISION_FP64, t_max=1.0, I=a.I,
+                    E=a.E, n_sims=64)
+    b = models.make_args(models.random_network(rng, n_species=70, n_reactions=12), E=[0], t_max=1.0, ita=3, U=64)
+    with pytest.raises(native.NativeError, match="N<=64"):
+        native.Plan(tables_of(b), b.rate_fn(b.c), b.x_0, precision=native.PRECISION_MIXED, t_max=1.0, I=b.I,
+                    E=b.E, n_sims=64)
 
 
 def _args():
@@ -64,14 +79,6 @@ def test_negative_rate_and_count_rejected(native):
         native.simulate(tabs, -rates, a.x_0, t_max=1.0, I=a.I, E=a.E, n_sims=8)
     with pytest.raises(native.NativeError, match="initial count"):
         native.simulate(tabs, rates, -a.x_0, t_max=1.0, I=a.I, E=a.E, n_sims=8)
-
-
-def test_network_too_large_is_reported(native):
-    rng = np.random.default_rng(1)
-    spn = models.random_network(rng, n_species=20, n_reactions=30)
-    a = models.make_args(spn, E=[0], t_max=1.0, ita=3, U=8)
-    with pytest.raises(native.NativeError, match="this build handles"):
-        native.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, t_max=1.0, I=a.I, E=a.E, n_sims=8)
 
 
 def test_pure_ssa_entry_rejected_on_tau_leaping_call(native):
