@@ -349,6 +349,71 @@ def other_workload(args):
     }
 
 
+def random_networks_workload(args, n_check=8, check_sims=512):
+    """256 random mass-action networks (5-50 species, 10-200 reactions) x 1024
+    simulations in one plan, mixed precision.  Networks beyond the FP64 size
+    classes have no FP64 GPU kernel, so accuracy is checked against the FP64
+    CPU oracle: for the first n_check networks, per species, the final-time
+    mean of the first check_sims simulations (flagged ones left out) as a
+    z-score against an oracle run of check_sims simulations."""
+    import torch
+    from py_stoch_amd import _native, workloads
+    from oracle import oracle
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    specs = workloads.random_networks()
+    O_list = [torch.empty((s.kappa, s.ita, s.n_sims), dtype=torch.int32, device=dev) for s in specs]
+    F_list = [torch.empty((s.n_sims,), dtype=torch.int32, device=dev) for s in specs]
+    print("bench: random networks, plan", file=sys.stderr, flush=True)
+    plan = _native.BatchPlan(workloads.jobs(specs, _native.PRECISION_MIXED, specialize=False))
+    print("bench: random networks, timed", file=sys.stderr, flush=True)
+    wall, kms, st = _time_batch(plan, O_list, args.steps, args.warmup, stream)
+    plan.execute([O.data_ptr() for O in O_list], [O.shape[-1] for O in O_list], [F.data_ptr() for F in F_list],
+                 stream.cuda_stream)
+    torch.cuda.synchronize()
+    print("bench: random networks, oracle check", file=sys.stderr, flush=True)
+    zmax, flag_frac = [], []
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t_cpu, ev_cpu = 0.0, 0
+    for s, O, F in list(zip(specs, O_list, F_list))[:n_check]:
+        kw = dict(s.kw)
+        kw["n_sims"] = check_sims
+        t0 = time.perf_counter()
+        Oc, Fc, sc = oracle.simulate(s.tables, s.rates, s.x0, threads=threads, **kw)
+        t_cpu += time.perf_counter() - t0
+        ev_cpu += sc["events"]
+        Og = O[:, -1, :check_sims].cpu().numpy().astype(np.float64)
+        Fg = F[:check_sims].cpu().numpy()
+        x, y = Og[:, Fg == 0], Oc[:, -1, Fc == 0].astype(np.float64)
+        flag_frac.append(float((Fg != 0).mean()))
+        if x.shape[-1] > 10 and y.shape[-1] > 10:
+            se = np.sqrt(x.var(axis=-1) / x.shape[-1] + y.var(axis=-1) / y.shape[-1])
+            ok = se > 0
+            zmax.append(float((np.abs(x.mean(axis=-1) - y.mean(axis=-1))[ok] / se[ok]).max()) if ok.any() else 0.0)
+    value = st["events"] * args.steps / wall
+    return {
+        "metric": "reaction events/s, 1 GPU (random-networks)", "value": value, "unit": "reaction events/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "random-networks", "networks": len(specs),
+                   "simulations": sum(s.n_sims for s in specs),
+                   "species_range": [min(int(s.tables["N"]) for s in specs), max(int(s.tables["N"]) for s in specs)],
+                   "reactions_range": [min(int(s.tables["M"]) for s in specs),
+                                       max(int(s.tables["M"]) for s in specs)]},
+        "kernel_ms": kms,
+        "steps_per_s": (st["leaps"] + st["ssa_steps"]) * args.steps / wall,
+        "counters": {k: v for k, v in st.items() if k != "kernel_ms"},
+        "step_budget": {"max_steps_per_simulation": int(specs[0].kw["max_steps"]),
+                        "flagged_simulations": int(st["flagged"])},
+        "accuracy_vs_oracle": {"networks_checked": len(zmax), "simulations_each": check_sims,
+                               "max_abs_z_per_network": zmax, "flagged_fraction": flag_frac},
+        "cpu_baseline": {"value": ev_cpu / t_cpu, "unit": "reaction events/s", "cores": threads, "kind": "port",
+                         "sample": "first %d networks x %d simulations, FP64 oracle (C, OpenMP), %.2f s"
+                                   % (n_check, check_sims, t_cpu)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -358,7 +423,8 @@ def main():
     ap.add_argument("--acc-sets", type=int, default=16,
                     help="independent sets of 65536 simulations per path for the accuracy check")
     ap.add_argument("--workload", default="lotka-volterra",
-                    choices=["lotka-volterra", "birth-death", "schlogl", "schlogl-ssa", "sweep"],
+                    choices=["lotka-volterra", "birth-death", "schlogl", "schlogl-ssa", "sweep",
+                             "random-networks"],
                     help="lotka-volterra (default) is the headline measurement; the others measure "
                          "the remaining configurations with the same JSON fields")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -371,6 +437,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "random-networks":
+        print(json.dumps(random_networks_workload(args)))
+        return
     if args.workload != "lotka-volterra":
         print(json.dumps(other_workload(args)))
         return

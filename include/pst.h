@@ -66,6 +66,8 @@ extern "C" {
 #define PST_FLAG_HALVING 2    /* tau halved 2000 times without a valid state   */
 #define PST_FLAG_TAU_INF 4    /* leap with a non-finite tau                    */
 #define PST_FLAG_PTRS 8       /* Poisson rejection loop limit reached          */
+#define PST_FLAG_STEPS 16     /* cfg.max_steps reached                          */
+#define PST_FLAG_STALL 32     /* tau > 0 but t + tau == t: the clock is stuck   */
 
 typedef struct pst_network {
   int32_t n_species;          /* N  (TlArgs.N)                               */
@@ -111,6 +113,11 @@ typedef struct pst_config {
                                  model's kernels (cuTauLeaping.py:34-35);
                                  0: table-driven kernel of the library      */
   int32_t reserved;           /* must be 0                                   */
+  int64_t max_steps;          /* 0: none; else at most this many leap
+                                 decisions + SSA steps per simulation, after
+                                 which it ends with PST_FLAG_STEPS (bounds the
+                                 run time of stiff or pathological networks;
+                                 the reference has no such bound) [0, 2^32)  */
 } pst_config;
 
 typedef struct pst_stats {

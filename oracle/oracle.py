@@ -25,6 +25,8 @@ FLAG_OVERFLOW = 1
 FLAG_HALVING = 2
 FLAG_TAU_INF = 4
 FLAG_PTRS = 8
+FLAG_STEPS = 16
+FLAG_STALL = 32
 
 
 class OrcNetwork(ctypes.Structure):
@@ -57,6 +59,7 @@ class OrcConfig(ctypes.Structure):
         ("sims_per_set", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
         ("sim_offset", ctypes.c_uint64),
+        ("max_steps", ctypes.c_int64),
     ]
 
 
@@ -154,7 +157,7 @@ def max_threads():
 
 def simulate(tables, rates, x0, method=METHOD_TAU_LEAPING, n_c=10, eta=0.03, ssa_burst=100,
              t_max=1.0, I=None, E=(0,), n_sims=1, sims_per_set=None, seed=0, sim_offset=0,
-             o_stride=None, threads=0):
+             o_stride=None, threads=0, max_steps=0):
     """Run the oracle.
 
     ``tables`` is a dict with int32 arrays ``A`` (a_size x 4), ``V`` (v_size x 4),
@@ -181,7 +184,7 @@ def simulate(tables, rates, x0, method=METHOD_TAU_LEAPING, n_c=10, eta=0.03, ssa
                      _p(H, ctypes.c_int32), _p(Ht, ctypes.c_int32), _p(po, ctypes.c_int32),
                      _p(ps, ctypes.c_int32))
     cfg = OrcConfig(method, n_c, eta, ssa_burst, float(t_max), I.shape[0], _p(I, ctypes.c_float),
-                    E.shape[0], _p(E, ctypes.c_int32), n_sims, sims_per_set, seed, sim_offset)
+                    E.shape[0], _p(E, ctypes.c_int32), n_sims, sims_per_set, seed, sim_offset, int(max_steps))
     O = np.zeros((E.shape[0], I.shape[0], o_stride), np.int32)
     flags = np.zeros(max(n_sims, 1), np.int32)
     st = OrcStats()

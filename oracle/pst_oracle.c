@@ -35,6 +35,8 @@
 /* per-simulation flag bits (identical meaning in the HIP library) */
 #define ORC_FLAG_OVERFLOW   1  /* a count or a Poisson mean exceeded 2^30   */
 #define ORC_FLAG_HALVING    2  /* tau halved ORC_MAX_HALVINGS times         */
+#define ORC_FLAG_STEPS     16  /* max_steps reached (not in the reference)  */
+#define ORC_FLAG_STALL     32  /* tau > 0 but t + tau == t (clock stuck)    */
 #define ORC_FLAG_TAU_INF    4  /* leap with a non-finite tau                */
 #define ORC_FLAG_PTRS       8  /* PTRS rejection loop limit                 */
 
@@ -296,6 +298,7 @@ typedef struct {
     int32_t sims_per_set; /* sims sharing one (rates, x0) row */
     uint64_t seed;
     uint64_t sim_offset;
+    int64_t max_steps;    /* 0: no limit; else leap decisions + SSA steps per simulation */
 } orc_config;
 
 typedef struct {
@@ -377,6 +380,7 @@ static int simulate_one(const orc_network* net, const double* k, const int32_t* 
     double t = 0.0;
     int F = 0;
     int flags = 0;
+    int64_t steps = 0;  /* leap decisions + SSA steps, for max_steps */
 
     /* I_rs: species with any Pre entry (P1_P2.py:338-346) */
     for (int i = 0; i < N; ++i) Irs[i] = 0;
@@ -385,6 +389,10 @@ static int simulate_one(const orc_network* net, const double* k, const int32_t* 
     if (c->method == 2) {
         /* kernel_Gillespie, gillespie.py:70-146 */
         while (t < c->t_max) {
+            if (c->max_steps > 0 && steps >= c->max_steps) {
+                flags |= ORC_FLAG_STEPS; fill_rest(c, O, U, sim, F, x, 1); return flags;
+            }
+            ++steps;
             for (int j = 0; j < M; ++j) a[j] = 0.0;
             propensities(net, k, x, a);
             double a0 = 0.0;
@@ -404,7 +412,11 @@ static int simulate_one(const orc_network* net, const double* k, const int32_t* 
                 if (rc >= rho) { j = r; break; }
             }
             if (j < 0) j = last;
-            t = t + tau;
+            {
+                const double t_old = t;
+                t = t + tau;
+                if (t == t_old && tau > 0.0) { flags |= ORC_FLAG_STALL; fill_rest(c, O, U, sim, F, x, 1); return flags; }
+            }
             while (t >= (double)c->I[F]) {
                 for (int q = 0; q < c->kappa; ++q)
                     O[((int64_t)q * c->ita + F) * U + sim] = x[c->E[q]];
@@ -431,6 +443,10 @@ static int simulate_one(const orc_network* net, const double* k, const int32_t* 
             Q = 0;   /* GILLESPIE macro, P1_P2.py:67-74 */
         } else {
             /* ---- kernel_P1_P2 (P1_P2.py:96-294) ---- */
+            if (c->max_steps > 0 && steps >= c->max_steps) {
+                flags |= ORC_FLAG_STEPS; fill_rest(c, O, U, sim, F, x, 1); return flags;
+            }
+            ++steps;
             for (int j = 0; j < M; ++j) a[j] = 0.0;
             propensities(net, k, x, a);
             double a0 = 0.0;
@@ -542,7 +558,11 @@ static int simulate_one(const orc_network* net, const double* k, const int32_t* 
                 st->events += ev;
                 st->leaps++;
                 /* time advance and SaveInterpolatedDynamics (P1_P2.py:261-286,542-584) */
-                t = t + tau;
+                {
+                    const double t_old = t;
+                    t = t + tau;
+                    if (t == t_old && tau > 0.0) { flags |= ORC_FLAG_STALL; fill_rest(c, O, U, sim, F, x, 1); return flags; }
+                }
                 int done = 0;
                 while (t >= (double)c->I[F]) {
                     double t0 = t - tau;
@@ -566,6 +586,10 @@ static int simulate_one(const orc_network* net, const double* k, const int32_t* 
         if (Q == 0) {
             /* ---- kernel_P3 (P3.py:55-161) ---- */
             for (int cnt = 0; cnt < c->ssa_burst; ++cnt) {
+                if (c->max_steps > 0 && steps >= c->max_steps) {
+                    flags |= ORC_FLAG_STEPS; fill_rest(c, O, U, sim, F, x, 1); return flags;
+                }
+                ++steps;
                 for (int j = 0; j < M; ++j) a[j] = 0.0;
                 propensities(net, k, x, a);
                 double a0 = 0.0;
@@ -586,7 +610,11 @@ static int simulate_one(const orc_network* net, const double* k, const int32_t* 
                     if (rc >= rho) { j = r; break; }
                 }
                 if (j < 0) j = last;
-                t = t + tau;
+                {
+                    const double t_old = t;
+                    t = t + tau;
+                    if (t == t_old && tau > 0.0) { flags |= ORC_FLAG_STALL; fill_rest(c, O, U, sim, F, x, 1); return flags; }
+                }
                 while (t >= (double)c->I[F]) {
                     /* SaveDynamics: the pre-reaction state (P3.py:165-177) */
                     for (int q = 0; q < c->kappa; ++q)

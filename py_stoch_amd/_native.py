@@ -24,6 +24,8 @@ FLAG_OVERFLOW = 1
 FLAG_HALVING = 2
 FLAG_TAU_INF = 4
 FLAG_PTRS = 8
+FLAG_STEPS = 16
+FLAG_STALL = 32
 
 _P_I32 = ctypes.POINTER(ctypes.c_int32)
 _P_F64 = ctypes.POINTER(ctypes.c_double)
@@ -66,6 +68,7 @@ class PstConfig(ctypes.Structure):
         ("sim_offset", ctypes.c_uint64),
         ("specialize", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("max_steps", ctypes.c_int64),
     ]
 
 
@@ -184,7 +187,7 @@ class _Marshalled:
     """Keeps the numpy arrays behind a PstNetwork/PstConfig alive."""
 
     def __init__(self, tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I, E,
-                 n_sims, sims_per_set, seed, sim_offset, specialize=False):
+                 n_sims, sims_per_set, seed, sim_offset, specialize=False, max_steps=0):
         N, M = int(tables["N"]), int(tables["M"])
         self.A = np.ascontiguousarray(tables["A"], np.int32).reshape(-1, 4)
         self.V = np.ascontiguousarray(tables["V"], np.int32).reshape(-1, 4)
@@ -219,19 +222,19 @@ class _Marshalled:
                              float(t_max), self.I.shape[0], self.E.shape[0],
                              _ptr(self.I, ctypes.c_float), _ptr(self.E, ctypes.c_int32),
                              int(n_sims), int(sims_per_set), int(seed), int(sim_offset),
-                             1 if specialize else 0, 0)
+                             1 if specialize else 0, 0, int(max_steps))
 
 
 def simulate(tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64, n_c=10,
              eta=0.03, ssa_burst=100, t_max=1.0, I=None, E=(0,), n_sims=1, sims_per_set=None,
-             seed=0, sim_offset=0, return_flags=True, specialize=False):
+             seed=0, sim_offset=0, return_flags=True, specialize=False, max_steps=0):
     """Run simulations on the GPU; returns (O[kappa][ita][n_sims] int32, flags, stats).
 
     specialize=True compiles (once per network and settings, cached) a kernel
     with the network built in instead of using the table-driven kernel."""
     L = load()
     m = _Marshalled(tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I, E,
-                    n_sims, sims_per_set, seed, sim_offset, specialize)
+                    n_sims, sims_per_set, seed, sim_offset, specialize, max_steps)
     O = np.empty((m.E.shape[0], m.I.shape[0], int(n_sims)), np.int32)
     flags = np.zeros(max(int(n_sims), 1), np.int32) if return_flags else None
     st = PstStats()
@@ -248,10 +251,10 @@ class Plan:
 
     def __init__(self, tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64,
                  n_c=10, eta=0.03, ssa_burst=100, t_max=1.0, I=None, E=(0,), n_sims=1,
-                 sims_per_set=None, seed=0, sim_offset=0, specialize=False):
+                 sims_per_set=None, seed=0, sim_offset=0, specialize=False, max_steps=0):
         L = load()
         self._m = _Marshalled(tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I,
-                              E, n_sims, sims_per_set, seed, sim_offset, specialize)
+                              E, n_sims, sims_per_set, seed, sim_offset, specialize, max_steps)
         self.n_sims = int(n_sims)
         self.kappa = self._m.E.shape[0]
         self.ita = self._m.I.shape[0]
@@ -296,9 +299,9 @@ class Job:
 
     def __init__(self, tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64, n_c=10,
                  eta=0.03, ssa_burst=100, t_max=1.0, I=None, E=(0,), n_sims=1, sims_per_set=None, seed=0,
-                 sim_offset=0, specialize=False):
+                 sim_offset=0, specialize=False, max_steps=0):
         self.m = _Marshalled(tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I, E, n_sims,
-                             sims_per_set, seed, sim_offset, specialize)
+                             sims_per_set, seed, sim_offset, specialize, max_steps)
         self.n_sims = int(n_sims)
         self.kappa = self.m.E.shape[0]
         self.ita = self.m.I.shape[0]

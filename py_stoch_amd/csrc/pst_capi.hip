@@ -123,6 +123,8 @@ int prepare(const pst_network* net, const pst_config* cfg, const double* rates, 
   const int64_t need_sets = cfg->n_sims == 0 ? 0 : ((int64_t)cfg->n_sims + cfg->sims_per_set - 1) / cfg->sims_per_set;
   if (n_sets < need_sets) return fail(PST_EINVAL, "%d parameter rows given, %lld needed", n_sets, (long long)need_sets);
   if (cfg->n_c < 0) return fail(PST_EINVAL, "n_c must be >= 0");
+  if (cfg->max_steps < 0 || cfg->max_steps > 0xffffffffll)
+    return fail(PST_EINVAL, "max_steps must be in [0, 2^32)");
   if (cfg->ssa_burst < 1) return fail(PST_EINVAL, "ssa_burst must be >= 1");
   if (!(cfg->eta > 0.0) || !std::isfinite(cfg->eta)) return fail(PST_EINVAL, "eta must be a positive finite number");
   if (cfg->method == PST_METHOD_PURE_SSA && !std::isfinite(cfg->t_max))
@@ -409,6 +411,7 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
     P.sims_per_set = J.cfg.sims_per_set;
     P.ssa_slice = kSsaSlice;
     P.n_rates = n_rates;
+    P.max_steps = (uint32_t)J.cfg.max_steps;
     P.eta = J.cfg.eta;
     P.t_max = J.cfg.t_max;
     P.seed = J.cfg.seed;

@@ -34,6 +34,8 @@ constexpr int FLAG_OVERFLOW = 1;
 constexpr int FLAG_HALVING = 2;
 constexpr int FLAG_TAU_INF = 4;
 constexpr int FLAG_PTRS = 8;
+constexpr int FLAG_STEPS = 16;  // per-simulation step budget reached
+constexpr int FLAG_STALL = 32;  // tau > 0 but t + tau == t: the clock is stuck
 constexpr int MAX_HALVINGS = 2000;
 constexpr int COUNT_LIMIT = 1073741824;  // 2^30
 constexpr double LAMBDA_LIMIT = 1.0e9;

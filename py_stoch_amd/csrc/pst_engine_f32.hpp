@@ -189,6 +189,7 @@ struct EngineF32 {
   int F;
   int flag;
   int burst;
+  uint32_t nsteps;  // leap decisions + SSA steps so far (step budget)
   int32_t* Ob;
 
   __device__ __forceinline__ void fill_rest(const Ref& tab, bool zero, Counters& cnt) const {
@@ -217,6 +218,7 @@ struct EngineF32 {
     tnext = (double)P.I[0];
     flag = 0;
     burst = 0;
+    nsteps = 0;
     Ob = P.O + sim;
     return P.method == 0 ? MODE_DECIDE : MODE_SSA;
   }
@@ -234,6 +236,13 @@ struct EngineF32 {
       fill_rest(tab, true, cnt);
       return MODE_DONE;
     }
+    // step budget, checked before every leap decision and every SSA step
+    if (P.max_steps != 0u && nsteps >= P.max_steps) {
+      fill_rest(tab, true, cnt);
+      flag = FLAG_STEPS;
+      return MODE_DONE;
+    }
+    nsteps += 1;
     // Philox blocks bc .. bc + NB, generated together as independent chains:
     // block bc serves the one-block draw (SSA step, or tau2 and the critical
     // reaction) and the first Poisson attempt takes NB consecutive blocks
@@ -297,6 +306,12 @@ struct EngineF32 {
       if (tau1 < 10.0f * fast_rcp(a0)) {
         ssa = true;
         burst = P.ssa_burst;
+        if (P.max_steps != 0u && nsteps >= P.max_steps) {
+          fill_rest(tab, true, cnt);
+          flag = FLAG_STEPS;
+          return MODE_DONE;
+        }
+        nsteps += 1;
       }
     }
     bool msk[MP];
@@ -368,6 +383,11 @@ struct EngineF32 {
     }
     const double t0 = t;
     t = t + (double)tau;
+    if (t == t0 && tau > 0.0f) {  // tau == 0 (a draw of u = 1) is a valid event
+      fill_rest(tab, true, cnt);
+      flag = FLAG_STALL;
+      return MODE_DONE;
+    }
     if (t >= tnext) {
       const float inv_gap = fast_rcp((float)(t - t0));
       while (t >= tnext) {

@@ -140,6 +140,7 @@ struct EngineF64 {
   int F;
   int flag;
   int burst;    // SSA steps left in the current fallback burst (method 0)
+  uint32_t nsteps;  // leap decisions + SSA steps so far (step budget)
   int32_t* Ob;  // &O[0][0][sim]
 
   __device__ __forceinline__ void fill_rest(const Ref& tab, bool zero, Counters& cnt) const {
@@ -168,6 +169,7 @@ struct EngineF64 {
     tnext = (double)P.I[0];
     flag = 0;
     burst = 0;
+    nsteps = 0;
     Ob = P.O + sim;
     return P.method == 0 ? MODE_DECIDE : MODE_SSA;
   }
@@ -191,6 +193,13 @@ struct EngineF64 {
       fill_rest(tab, true, cnt);
       return MODE_DONE;
     }
+    // step budget, checked before every leap decision and every SSA step
+    if (P.max_steps != 0u && nsteps >= P.max_steps) {
+      fill_rest(tab, true, cnt);
+      flag = FLAG_STEPS;
+      return MODE_DONE;
+    }
+    nsteps += 1;
     double a[MP];
     propensities_f64<Ref>(tab, k, x, a);
     double a0 = 0.0;
@@ -247,6 +256,12 @@ struct EngineF64 {
       if (tau1 < 10.0 / a0) {
         ssa = true;
         burst = P.ssa_burst;
+        if (P.max_steps != 0u && nsteps >= P.max_steps) {
+          fill_rest(tab, true, cnt);
+          flag = FLAG_STEPS;
+          return MODE_DONE;
+        }
+        nsteps += 1;
       }
     }
     // the one-block draw: SSA step, or tau2 + SingleCriticalReaction (P1_P2.py:175-220)
@@ -322,7 +337,13 @@ struct EngineF64 {
     // time advance; samples: sample-and-hold of the pre-step state for SSA
     // (SaveDynamics, P3.py:165-177), interpolation for a leap
     // (SaveInterpolatedDynamics, P1_P2.py:261-286,542-584)
+    const double t_prev = t;
     t = t + tau;
+    if (t == t_prev && tau > 0.0) {  // tau == 0 (a draw of u = 1) is a valid event
+      fill_rest(tab, true, cnt);
+      flag = FLAG_STALL;
+      return MODE_DONE;
+    }
     while (t >= tnext) {
       const double t0 = t - tau;
       const float record_time = P.I[F];

@@ -107,6 +107,7 @@ struct EngineWave {
   uint64_t bc;
   double t, tnext;
   int F, flag, burst, mode;
+  uint32_t nsteps;  // leap decisions + SSA steps so far (step budget)
   int64_t sim;
 
   // write samples F..ita-1: x of the recorded species, or zeros
@@ -138,6 +139,7 @@ struct EngineWave {
     tnext = (double)P.I[0];
     flag = 0;
     burst = 0;
+    nsteps = 0;
     mode = P.method == 0 ? MODE_DECIDE : MODE_SSA;
   }
 
@@ -152,6 +154,12 @@ struct EngineWave {
       fill_rest(J, L, lane, true, cnt);
       return true;
     }
+    if (P.max_steps != 0u && nsteps >= P.max_steps) {
+      fill_rest(J, L, lane, true, cnt);
+      flag = FLAG_STEPS;
+      return true;
+    }
+    nsteps += 1;
     // propensities of the lane's reactions
     float a[WAVE_R];
 #pragma unroll
@@ -230,6 +238,12 @@ struct EngineWave {
         ssa = true;
         mode = MODE_SSA;
         burst = P.ssa_burst;
+        if (P.max_steps != 0u && nsteps >= P.max_steps) {
+          fill_rest(J, L, lane, true, cnt);
+          flag = FLAG_STEPS;
+          return true;
+        }
+        nsteps += 1;
       }
     }
     // one-block draw: SSA step, or tau2 and the critical reaction
@@ -304,6 +318,11 @@ struct EngineWave {
     // time advance and samples (sample-and-hold for SSA, interpolation for a leap)
     const double t0 = t;
     t = t + (double)tau;
+    if (t == t0 && tau > 0.0f) {  // tau == 0 (a draw of u = 1) is a valid event
+      fill_rest(J, L, lane, true, cnt);
+      flag = FLAG_STALL;
+      return true;
+    }
     if (t >= tnext) {
       const float inv_gap = fast_rcp((float)(t - t0));
       while (t >= tnext) {

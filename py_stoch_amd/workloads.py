@@ -21,7 +21,12 @@ states and run settings) that ``_native.Job`` / ``_native.Plan`` accept:
   random_networks    256 random mass-action networks from
                      numpy.random.default_rng(5) (5-50 species, 10-200
                      reactions), 1024 simulations each, tau-leaping,
-                     t_end = 5, 50 samples, every species recorded, seed 5
+                     t_end = 5, 50 samples, every species recorded, seed 5;
+                     rate constants span five decades, so many networks are
+                     stiff (median ~1e6 leaps per simulation to t_end, some
+                     above 1e7): each simulation gets a step budget
+                     (max_steps, 20000 by default) and ends flagged when it
+                     runs out
 
 Parameter sweeps perturb the kinetic-law parameters (``TlArgs.c``), which
 the reference's sim_maker sweeps (sim_maker.py:6-43).
@@ -98,14 +103,47 @@ def sweep(n_sets=4096, sims_per_set=256):
     return out
 
 
-def random_networks(n_networks=256, sims=1024, max_species=50, max_reactions=200):
+def mass_action_tables(spn):
+    """tables_of(TlParser.parse(spn)) for a network whose every hazard is
+    c_j / prod_i Pre_ij! * prod_i x_i (x_i - 1) ... (x_i - Pre_ij + 1) (the form
+    models.random_network writes), built from Pre / Post directly instead of
+    through sympy; returns (tables, per-reaction rates)."""
+    from .mod.parser.parser import Parser
+    pre = np.asarray(spn.Pre, np.int64)
+    post = np.asarray(spn.Post, np.int64)
+    N, M = pre.shape
+    order = pre.sum(axis=0)                      # reaction orders (tl_parser.py:80-95)
+    H = np.zeros(N, np.int32)
+    Ht = np.zeros(N, np.int32)
+    for i in range(N):
+        js = np.nonzero(pre[i])[0]               # reactions whose hazard names species i
+        if js.size:
+            H[i] = order[js].max()
+            Ht[i] = pre[i, js].max()
+    fact = np.ones(M)
+    for i in range(N):
+        for j in range(M):
+            for q in range(2, int(pre[i, j]) + 1):
+                fact[j] *= q
+    A, _ = Parser.flatten_matrix(pre)
+    V, _ = Parser.flatten_matrix(post - pre)
+    tabs = {"N": N, "M": M, "A": np.asarray(A, np.int32).reshape(-1, 4), "V": np.asarray(V, np.int32).reshape(-1, 4),
+            "H": H, "H_type": Ht, "prop_order": pre.astype(np.int32), "prop_step": np.ones((N, M), np.int32),
+            "hazard_c": None, "n_params": 0}
+    return tabs, np.asarray(spn.c, np.float64) / fact
+
+
+def random_networks(n_networks=256, sims=1024, max_species=50, max_reactions=200, max_steps=20000):
     rng = np.random.default_rng(5)
     out = []
     for n in range(n_networks):
         spn = models.random_network(rng, max_species=max_species, max_reactions=max_reactions)
-        a = models.make_args(spn, E=list(range(len(spn.M))), t_max=5.0, ita=50, U=sims)
-        out.append(Spec("random-%d" % n, tables_of(a), a.rate_fn(a.c), a.x_0, method=0, t_max=5.0, I=a.I,
-                        E=a.E, n_sims=sims, seed=5, n_c=10, eta=0.03, ssa_burst=100))
+        tabs, rates = mass_action_tables(spn)
+        N = tabs["N"]
+        I = np.linspace(0, 5.0, num=50).astype(np.float32)
+        out.append(Spec("random-%d" % n, tabs, rates, np.asarray(spn.M, np.int32), method=0, t_max=5.0, I=I,
+                        E=np.arange(N, dtype=np.int32), n_sims=sims, seed=5, n_c=10, eta=0.03, ssa_burst=100,
+                        max_steps=max_steps))
     return out
 
 

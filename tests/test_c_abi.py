@@ -59,6 +59,7 @@ def _args():
     (lambda t, kw: kw.update(n_c=-1), "n_c"),
     (lambda t, kw: kw.update(eta=0.0), "eta"),
     (lambda t, kw: kw.update(ssa_burst=0), "ssa_burst"),
+    (lambda t, kw: kw.update(max_steps=-1), "max_steps"),
     (lambda t, kw: kw.update(precision=7), "precision"),
     (lambda t, kw: t.update(V=np.array([[0, 0, 1, 0], [0, 0, -1, 0]])), "two entries"),
     (lambda t, kw: t.update(V=np.array([[1, 1, 1, 0], [0, 0, -1, 0]])), "species-major"),

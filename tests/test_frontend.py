@@ -214,3 +214,19 @@ def test_timer():
     with Timer() as t:
         sum(range(1000))
     assert t.msecs >= 0.0
+
+
+def test_random_network_fast_tables_equal_parser_tables():
+    """The direct table builder used for the random-network workload gives the
+    same tables and rates as the SBML-style parse (TlParser + mass-action
+    analysis) of the same networks."""
+    from py_stoch_amd import workloads
+    rng = np.random.default_rng(9)
+    for _ in range(4):
+        spn = models.random_network(rng, max_species=12, max_reactions=25)
+        a = models.make_args(spn, E=[0], t_max=1.0, ita=3, U=8)
+        ref = tables_of(a)
+        tabs, rates = workloads.mass_action_tables(spn)
+        for key in ("N", "M", "A", "V", "H", "H_type", "prop_order"):
+            assert np.array_equal(np.asarray(tabs[key]), np.asarray(ref[key])), key
+        np.testing.assert_allclose(rates, a.rate_fn(a.c), rtol=1e-15)

@@ -112,3 +112,21 @@ def test_persistent_refill_many_sims_per_lane(native, oracle_mod):
     Oc, Fc, sc = oracle_mod.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, **kw)
     assert np.array_equal(O, Oc)
     assert st["events"] == sc["events"]
+
+
+@pytest.mark.parametrize("specialize", [False, True])
+def test_step_budget_bit_exact(native, oracle_mod, specialize):
+    """The per-simulation step budget ends simulations at the same step, with
+    the same samples and flags, as the oracle (Lotka-Volterra: the budget hits
+    during leaps; Schlogl: also during SSA bursts)."""
+    for spn, E, t_max, ita, budget in ((models.lotka_volterra(), [0, 1], 10.0, 101, 300),
+                                       (models.schlogl(), [0], 10.0, 51, 180)):
+        a = models.make_args(spn, E=E, t_max=t_max, ita=ita, U=512)
+        kw = dict(method=0, t_max=t_max, I=a.I, E=a.E, n_sims=512, seed=3, max_steps=budget)
+        print("case: budget %d specialize=%s" % (budget, specialize), flush=True)
+        O_g, f_g, s_g = native.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, specialize=specialize, **kw)
+        O_c, f_c, s_c = oracle_mod.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, **kw)
+        assert np.array_equal(f_g, f_c) and (f_c & native.FLAG_STEPS).any()
+        assert np.array_equal(O_g, O_c)
+        for k in ("events", "leaps", "ssa_steps", "flagged"):
+            assert s_g[k] == s_c[k], k

@@ -159,3 +159,22 @@ def test_thread_count_does_not_change_results(oracle_mod):
     _, O1, _, s1 = _run(oracle_mod, spn, [0], 2.0, 5, 256, oracle_mod.METHOD_TAU_LEAPING, seed=4, threads=1)
     _, O8, _, s8 = _run(oracle_mod, spn, [0], 2.0, 5, 256, oracle_mod.METHOD_TAU_LEAPING, seed=4, threads=8)
     assert np.array_equal(O1, O8) and s1 == s8
+
+
+def test_step_budget_ends_simulations(oracle_mod):
+    """max_steps: every simulation stops after that many leap decisions + SSA
+    steps with the STEPS flag; up to the stop its trajectory is the unbudgeted
+    one (the budget changes nothing before it is reached)."""
+    a = models.make_args(models.lotka_volterra(), E=[0, 1], t_max=10.0, ita=101, U=64)
+    kw = dict(t_max=10.0, I=a.I, E=a.E, n_sims=64, seed=1)
+    O, F, st = oracle_mod.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, max_steps=300, **kw)
+    O2, F2, st2 = oracle_mod.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, **kw)
+    assert (F == oracle_mod.FLAG_STEPS).all() and (F2 == 0).all()
+    assert st["leaps"] + st["ssa_steps"] <= 64 * 300
+    # up to the sample where the budget ran out, each trajectory is the unbudgeted
+    # one; from there on the samples are zero
+    for s in range(64):
+        nz = np.nonzero(O[:, :, s].any(axis=0))[0]
+        f = nz[-1] + 1
+        assert 1 <= f < O.shape[1] and (O[:, f:, s] == 0).all()
+        assert np.array_equal(O[:, :f, s], O2[:, :f, s])
