@@ -42,7 +42,7 @@ int hip_fail(hipError_t e, const char* what) {
 // largest network of any kernel (the large-network kernel, mixed precision)
 constexpr int kMaxSpecies = pst::WAVE_MAX_N;
 constexpr int kMaxReactions = pst::WAVE_MAX_M;
-constexpr int kSsaSlice = 4;
+constexpr int kSsaSlice = 7;
 constexpr int kMaxParams = 256;
 
 // A compiled hazard may use X(i), C(k), R(v), numbers, + - * / ( ) , and these

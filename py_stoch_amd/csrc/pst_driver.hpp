@@ -12,9 +12,12 @@
 // Per lane the driver keeps a mode -- DECIDE (next step is a leap decision,
 // P1_P2.py:96-294), SSA (inside a P3 burst, P3.py:71-151, or pure SSA) or
 // DONE.  Every pass of its loop runs one step for every active lane (the
-// engine's step() serves both modes with shared code), then, if at least
-// `ssa_lanes_min` lanes are in an SSA burst, up to `ssa_slice` further SSA-only
-// steps for them.  Every simulation executes exactly the reference's sequence
+// engine's step() serves both modes with shared code), then, while at least
+// SSA_LANES_MIN lanes are in an SSA burst, up to `ssa_slice` further SSA-only
+// steps for them: a wavefront lasts as long as its slowest simulation, and a
+// burst taken one step per pass would add 100 passes to that simulation's
+// path (8 steps per pass: Lotka-Volterra 81 -> 70 ms, Schlogl 110 -> 76 ms).
+// Every simulation executes exactly the reference's sequence
 // (decision, burst of ssa_burst SSA steps, decision, ...); only the
 // interleaving of different simulations within a wavefront changes.
 #pragma once
@@ -52,7 +55,7 @@ __device__ __forceinline__ int32_t pick_i(const int32_t (&x)[NP], int idx) {
 //   int  start(job, sim)            -> initial mode, state loaded
 //   int  step<SSA_ONLY>(job, cnt, m) -> next mode (DONE may set flag)
 //   int  flag
-constexpr int SSA_LANES_MIN = 32;
+constexpr int SSA_LANES_MIN = 1;
 
 template <class Engine, class Ref>
 __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__ jobs, int n_jobs,
@@ -108,7 +111,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
           cnt.flagged += (eng.flag != 0);
         }
       }
-      // SSA-only steps while enough lanes are inside a burst
+      // SSA-only steps while lanes are inside a burst
       const int slice = jr.p().ssa_slice;
       for (int s = 0; s < slice; ++s) {
         if (__builtin_popcountll(__builtin_amdgcn_ballot_w64(mode == MODE_SSA)) < SSA_LANES_MIN) break;
