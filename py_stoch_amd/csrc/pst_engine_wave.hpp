@@ -60,6 +60,7 @@ struct WaveLds {
   long long xn[WAVE_MAX_N];  // tentative state (x + V K)
   int32_t x[WAVE_MAX_N];     // current state
   float anc[WAVE_MAX_M];     // propensity of non-critical reactions, 0 for critical ones
+  int32_t K[WAVE_MAX_M];     // firings of the current leap attempt
   uint8_t crit[WAVE_MAX_M];
 };
 
@@ -103,6 +104,7 @@ __device__ __forceinline__ double wave_scan_d(double v, int lane) {
 // members are wavefront-uniform except the per-lane rate constants.
 struct EngineWave {
   float k[WAVE_R];  // rates of the lane's reactions
+
   Key key;
   uint64_t bc;
   double t, tnext;
@@ -166,7 +168,7 @@ struct EngineWave {
     for (int r = 0; r < WAVE_R; ++r) {
       const int j = lane + 64 * r;
       float v = 0.0f;
-      if (j < M) {
+      if (64 * r < M && j < M) {  // whole slots past M are skipped by the wavefront
         v = k[r];
         const int nr = T.nr[j];
         for (int e = 0; e < nr; ++e) {
@@ -200,7 +202,7 @@ struct EngineWave {
       for (int r = 0; r < WAVE_R; ++r) {
         const int j = lane + 64 * r;
         bool c = false;
-        if (j < M) {
+        if (64 * r < M && j < M) {
           const int nv = T.nv[j];
           for (int e = 0; e < nv; ++e) {
             const int v = T.vv[j][e];
@@ -275,6 +277,7 @@ struct EngineWave {
       int last = -1;
 #pragma unroll
       for (int r = 0; r < WAVE_R; ++r) {
+        if (64 * r >= M) break;  // wavefront-uniform
         const int j = lane + 64 * r;
         const bool m = (j < M) && (ssa || crit[r]);
         const double v = m ? (double)a[r] : 0.0;
@@ -304,7 +307,7 @@ struct EngineWave {
         return true;
       }
       long long ev = 0;
-      const int st = leap(J, L, lane, cnt, a, crit, sel, tau, ev);
+      const int st = leap(J, L, lane, cnt, sel, tau, ev);
       if (st != 0) {
         fill_rest(J, L, lane, true, cnt);
         flag = st;
@@ -360,22 +363,23 @@ struct EngineWave {
     return false;
   }
 
-  // Poisson leap with halving; x' left in L.xn.  Returns 0 or a flag.
-  __device__ int leap(const WaveJob& J, WaveLds& L, int lane, Counters& cnt, const float (&a)[WAVE_R],
-                      const bool (&crit)[WAVE_R], int sel, float tau, long long& ev) {
+  // Poisson leap with halving; x' left in L.xn.  Returns 0 or a flag.  The
+  // lane's reaction slots are walked in a rolled loop with the counts in LDS,
+  // so the sampler's code (and its registers) exists once.
+  __device__ int leap(const WaveJob& J, WaveLds& L, int lane, Counters& cnt, int sel, float tau, long long& ev) {
     const JobParams& P = J.p;
     const WaveTab& T = J.t;
     const int M = P.M, N = P.N;
     for (int h = 0; h < MAX_HALVINGS; ++h) {
       const uint64_t grp = bc;
       bc += (uint64_t)M;
-      int32_t K[WAVE_R];
       bool bad = false, fail = false;
-#pragma unroll
+#pragma unroll 1
       for (int r = 0; r < WAVE_R; ++r) {
         const int j = lane + 64 * r;
+        if (j >= M) continue;
         int32_t kk = (j == sel) ? 1 : 0;
-        const float l = (j < M && !crit[r]) ? a[r] * tau : 0.0f;
+        const float l = L.anc[j] * tau;  // 0 for critical reactions
         bad = bad || (l >= (float)LAMBDA_LIMIT);
         if (l > 0.0f && !bad) {
           const Words4 bw = key.block(grp + (uint64_t)j, 0);
@@ -415,7 +419,7 @@ struct EngineWave {
             else kk = (int32_t)d.k;
           }
         }
-        K[r] = kk;
+        L.K[j] = kk;
       }
       if (__builtin_amdgcn_ballot_w64(bad) != 0ull) return FLAG_OVERFLOW;
       if (__builtin_amdgcn_ballot_w64(fail) != 0ull) return FLAG_PTRS;
@@ -425,12 +429,15 @@ struct EngineWave {
       long long ev_lane = 0;
 #pragma unroll
       for (int r = 0; r < WAVE_R; ++r) {
+        if (64 * r >= M) break;  // wavefront-uniform
         const int j = lane + 64 * r;
-        if (j < M && K[r] != 0) {
-          ev_lane += K[r];
+        if (j >= M) continue;
+        const int32_t kk = L.K[j];
+        if (kk != 0) {
+          ev_lane += kk;
           const int nv = T.nv[j];
           for (int e = 0; e < nv; ++e)
-            atomicAdd((unsigned long long*)&L.xn[T.vs[j][e]], (unsigned long long)((long long)K[r] * T.vv[j][e]));
+            atomicAdd((unsigned long long*)&L.xn[T.vs[j][e]], (unsigned long long)((long long)kk * T.vv[j][e]));
         }
       }
       wave_lds_sync();
