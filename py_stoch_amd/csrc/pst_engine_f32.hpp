@@ -243,16 +243,20 @@ struct EngineF32 {
       return MODE_DONE;
     }
     nsteps += 1;
-    // Philox blocks bc .. bc + NB, generated together as independent chains:
-    // block bc serves the one-block draw (SSA step, or tau2 and the critical
-    // reaction) and the first Poisson attempt takes NB consecutive blocks
-    // after it, or from bc when there is no one-block draw.
+    // Philox blocks bc .. bc + NB - 1, generated together as independent
+    // chains: block bc serves the one-block draw (SSA step, or tau2 and the
+    // critical reaction) and the first Poisson attempt takes NB consecutive
+    // blocks after it, or from bc when there is no one-block draw; block
+    // bc + NB is generated at the leap, only if a lane of the wavefront made
+    // a one-block draw.
     constexpr int NB = (MP + 1) / 2;
+    const uint64_t bc0 = bc;
     Words4 B[NB + 1];
     B[0] = key.block(bc, 0);
+    B[NB] = Words4{0u, 0u, 0u, 0u};
     if (!SSA_ONLY) {
 #pragma unroll
-      for (int r = 1; r <= NB; ++r)
+      for (int r = 1; r < NB; ++r)
         if (2 * (r - 1) < M) B[r] = key.block(bc + (uint64_t)r, 0);
     }
     float a[MP];
@@ -369,6 +373,7 @@ struct EngineF32 {
       int32_t K[MP];
 #pragma unroll
       for (int j = 0; j < MP; ++j) K[j] = (sel == j) ? 1 : 0;
+      if (wave_any(one)) B[NB] = key.block(bc0 + (uint64_t)NB, 0);
       const int st = leap(tab, cnt, a, crit, tau, K, xn, B, one);
       if (st != 0) {
         fill_rest(tab, true, cnt);
@@ -489,52 +494,88 @@ struct EngineF32 {
       cnt.w.uniforms += 2u * (unsigned int)__builtin_popcount(drawn);
       if (bad) return FLAG_OVERFLOW;
       // sequential-search inversion (0 < lambda < 10) for all such reactions
-      // in lock-step: after `it` trips every live search is at k = it
+      // in lock-step, two search steps per trip: a live search at k = it
+      // steps to it + 1 and it + 2 and stops at the first k with u <= F(k);
+      // p and F past a search's end are computed but never read
       if (inv != 0u) {
         float p[MP], Fc[MP], u[MP];
-        unsigned act = 0;
+        bool live[MP];
+        bool any_live = false;
 #pragma unroll
         for (int j = 0; j < MP; ++j) {
           const bool on = (inv >> j) & 1u;
           u[j] = u24((j & 1) ? W[j >> 1].w2 : W[j >> 1].w0);
           p[j] = fast_exp(-lam[j]);
           Fc[j] = p[j];
-          act |= (on && u[j] > Fc[j]) ? (1u << j) : 0u;
+          live[j] = on && u[j] > Fc[j];
+          any_live = any_live | live[j];
         }
         cnt.w.uniforms += (unsigned int)__builtin_popcount(inv);
-        int it = 0;
-        while (act != 0u && it < INV_MAX_K) {
-          it += 1;
-          const float rk = fast_rcp((float)it);
+        for (int it = 0; it < INV_MAX_K && wave_any(any_live); it += 2) {
+          const float r1 = fast_rcp((float)(it + 1));
+          const float r2 = fast_rcp((float)(it + 2));
+          any_live = false;
 #pragma unroll
           for (int j = 0; j < MP; ++j) {
-            const bool on = (act >> j) & 1u;
-            const float pn = p[j] * lam[j] * rk;
-            const float Fn = Fc[j] + pn;
-            p[j] = on ? pn : p[j];
-            Fc[j] = on ? Fn : Fc[j];
-            K[j] = on ? it : K[j];
-            act = (on && !(u[j] > Fn)) ? (act & ~(1u << j)) : act;
+            const float p1 = p[j] * lam[j] * r1;
+            const float F1 = Fc[j] + p1;
+            const float p2 = p1 * lam[j] * r2;
+            const float F2 = F1 + p2;
+            K[j] = live[j] ? ((u[j] > F1) ? it + 2 : it + 1) : K[j];
+            live[j] = live[j] && (u[j] > F2);
+            any_live = any_live | live[j];
+            p[j] = p2;
+            Fc[j] = F2;
           }
         }
 #pragma unroll
         for (int j = 0; j < MP; ++j)
           if ((inv >> j) & 1u) cnt.w.inv += (unsigned int)K[j];
       }
-      // draws rejected at iteration 0 (and means of 2^22 and above, drawn in FP64)
-      while (pend != 0u) {
-        const int j = __builtin_ctz(pend);
-        pend &= pend - 1u;
-        const float l = pick_m<MP>(lam, j);
-        const Draw dr = (l >= BIG_LAMBDA_F32)
-                            ? ptrs_big_f64(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1, (double)l)
-                            : ptrs_rest_f32(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1, 1, l);
+      // means of 2^22 and above: FP64 PTRS, one reaction at a time (rare)
+      unsigned bigp = 0;
+#pragma unroll
+      for (int j = 0; j < MP; ++j) bigp |= (((pend >> j) & 1u) && lam[j] >= BIG_LAMBDA_F32) ? (1u << j) : 0u;
+      pend &= ~bigp;
+      while (bigp != 0u) {
+        const int j = __builtin_ctz(bigp);
+        bigp &= bigp - 1u;
+        const Draw dr = ptrs_big_f64(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1,
+                                     (double)pick_m<MP>(lam, j));
         cnt.w.ptrs += dr.iters;
         cnt.w.uniforms += 2 * dr.iters;
-        const int64_t kk = dr.k;
-        if (kk < 0) return FLAG_PTRS;
+        if (dr.k < 0) return FLAG_PTRS;
 #pragma unroll
-        for (int r = 0; r < MP; ++r) K[r] = (r == j) ? (int32_t)kk : K[r];
+        for (int r = 0; r < MP; ++r) K[r] = (r == j) ? (int32_t)dr.k : K[r];
+      }
+      // FP32 draws rejected at iteration 0: iteration `it` of reaction j uses
+      // half j % 2 of block (grp + j / 2, it).  Each lane works through its
+      // pending reactions in index order and moves to the next one as soon as
+      // a draw is accepted, so the wavefront runs the largest per-lane total
+      // of retries instead of one retry loop per pending reaction.
+      {
+        uint32_t it = 1;
+        bool fail = false;
+        while (pend != 0u) {
+          const int j = __builtin_ctz(pend);
+          const PtrsF32 Pt = ptrs_setup_f32(pick_m<MP>(lam, j));
+          const Words4 bw = key.block(grp + (uint64_t)(j >> 1), it);
+          const float U = u24((j & 1) ? bw.w2 : bw.w0) - 0.5f;
+          const float V = u24((j & 1) ? bw.w3 : bw.w1);
+          cnt.w.ptrs += 1;
+          cnt.w.uniforms += 2;
+          float kf, us;
+          if (ptrs_fast_f32(Pt, U, V, kf, us) || ptrs_slow_f32(Pt, V, kf, us)) {
+#pragma unroll
+            for (int r = 0; r < MP; ++r) K[r] = (r == j) ? (int32_t)kf : K[r];
+            pend &= pend - 1u;
+            it = 1;
+          } else if (++it >= (uint32_t)PTRS_MAX_ITERS) {
+            fail = true;
+            pend = 0u;
+          }
+        }
+        if (fail) return FLAG_PTRS;
       }
       bool ok = true, big = false;
 #pragma unroll
