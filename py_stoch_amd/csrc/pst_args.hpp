@@ -44,6 +44,8 @@ struct JobParams {
   int32_t ssa_slice;    // SSA steps per pass while other lanes wait to decide (launch tuning)
   int32_t n_rates;      // length of a rates row: M, or the parameter count of compiled hazards
   uint32_t max_steps;   // 0: none; else leap decisions + SSA steps per simulation
+  int32_t lanes;        // simulations per wavefront, 1..64 (launch tuning)
+  int32_t reserved;
   double eta;           // tl_parser.py:44
   double t_max;
   uint64_t seed;

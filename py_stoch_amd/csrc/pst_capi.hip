@@ -244,6 +244,23 @@ void pack_jobs(const std::vector<PlanJob>& jobs, int cls, std::vector<unsigned c
   *count = (int)v.size();
 }
 
+// Simulations per wavefront for one launch (jobs `ids`): 64 unless the launch
+// holds fewer than 64 simulations per SIMD, then the smallest power of two
+// that still needs no more wavefronts than the chip has SIMDs.  A wavefront
+// issues the same instructions whatever its lane count, so spreading a small
+// launch over every SIMD costs nothing per wavefront and leaves each one with
+// fewer divergent paths and a shorter slowest-lane tail.
+int lanes_for(const std::vector<PlanJob>& jobs, const std::vector<int>& ids, int simds) {
+  auto need = [&](int L) {
+    int64_t n = 0;
+    for (int q : ids) n += ((int64_t)jobs[q].p.n_sims + L - 1) / L;
+    return n;
+  };
+  int L = 64;
+  while (L > 1 && need(L / 2) <= (int64_t)simds) L /= 2;
+  return L;
+}
+
 }  // namespace
 
 enum { KIND_TABLE = 0, KIND_FIXED = 1, KIND_WAVE = 2 };
@@ -410,6 +427,7 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
     P.n_sims = J.cfg.n_sims;
     P.sims_per_set = J.cfg.sims_per_set;
     P.ssa_slice = kSsaSlice;
+    P.lanes = 64;
     P.n_rates = n_rates;
     P.max_steps = (uint32_t)J.cfg.max_steps;
     P.eta = J.cfg.eta;
@@ -483,6 +501,19 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
   }
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
+  // simulations per wavefront, per launch
+  const int simds = p->cu_count * 4;
+  for (int c = 0; c < pst::NUM_CLASSES; ++c) {
+    std::vector<int> ids;
+    for (int q = 0; q < n_jobs; ++q)
+      if (p->jobs[q].cls == c) ids.push_back(q);
+    const int L = lanes_for(p->jobs, ids, simds);
+    for (int q : ids) p->jobs[q].p.lanes = L;
+  }
+  for (FixedGroup& g : p->fixed) {
+    const int L = lanes_for(p->jobs, g.jobs, simds);
+    for (int q : g.jobs) p->jobs[q].p.lanes = L;
+  }
   // job descriptors, one device array per size class
   int count[pst::NUM_CLASSES] = {0, 0, 0};
   for (int c = 0; c < pst::NUM_CLASSES; ++c) {
@@ -562,7 +593,7 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
     if (count[c] == 0) continue;
     int64_t need = 0;
     for (const PlanJob& pj : p->jobs)
-      if (pj.cls == c) need += ((int64_t)pj.p.n_sims + 63) / 64;
+      if (pj.cls == c) need += ((int64_t)pj.p.n_sims + pj.p.lanes - 1) / pj.p.lanes;
     const int bpc = (p->precision == PST_PRECISION_MIXED) ? pst::blocks_per_cu_mixed(c) : pst::blocks_per_cu_fp64(c);
     const int64_t resident = (int64_t)bpc * p->cu_count * 4;
     const int waves = (int)(need < resident ? need : resident);
@@ -576,7 +607,7 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
   }
   for (FixedGroup& g : p->fixed) {
     int64_t need = 0;
-    for (int q : g.jobs) need += ((int64_t)p->jobs[q].p.n_sims + 63) / 64;
+    for (int q : g.jobs) need += ((int64_t)p->jobs[q].p.n_sims + p->jobs[q].p.lanes - 1) / p->jobs[q].p.lanes;
     const int64_t resident = (int64_t)g.k.blocks_per_cu * p->cu_count * 4;
     const int waves = (int)(need < resident ? need : resident);
     const int cnt = (int)g.jobs.size();

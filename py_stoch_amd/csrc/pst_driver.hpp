@@ -3,11 +3,13 @@
 //
 // A launch processes a list of jobs (network + run settings + parameter rows
 // + output array).  Each wavefront works on one job at a time, so the network
-// table is uniform across its lanes; lanes take simulations of that job from
-// a device work queue (one atomic per wavefront per refill) and take the next
-// one as soon as theirs ends, so uneven step counts do not leave lanes idle
-// until the job's queue is empty.  When the job is exhausted and every lane of
-// the wavefront has finished, the wavefront moves to the next job.
+// table is uniform across its lanes; its first `lanes` lanes (JobParams; 64
+// unless the launch has fewer simulations than 64 per SIMD) take simulations
+// of that job from a device work queue (one atomic per wavefront per refill)
+// and take the next one as soon as theirs ends, so uneven step counts do not
+// leave lanes idle until the job's queue is empty.  When the job is exhausted
+// and every lane of the wavefront has finished, the wavefront moves to the
+// next job.
 //
 // Per lane the driver keeps a mode -- DECIDE (next step is a leap decision,
 // P1_P2.py:96-294), SSA (inside a P3 burst, P3.py:71-151, or pure SSA) or
@@ -17,6 +19,7 @@
 // steps for them: a wavefront lasts as long as its slowest simulation, and a
 // burst taken one step per pass would add 100 passes to that simulation's
 // path (8 steps per pass: Lotka-Volterra 81 -> 70 ms, Schlogl 110 -> 76 ms).
+// A pass in which no lane decides runs up to 1 + ssa_slice SSA steps.
 // Every simulation executes exactly the reference's sequence
 // (decision, burst of ssa_burst SSA steps, decision, ...); only the
 // interleaving of different simulations within a wavefront changes.
@@ -73,7 +76,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
 
   for (;;) {
     // ---- refill lanes whose simulation ended from the job's queue ----
-    const bool want = (mode == MODE_DONE) && !exhausted;
+    const bool want = (mode == MODE_DONE) && !exhausted && lane < jr.p().lanes;
     const unsigned long long wmask = __builtin_amdgcn_ballot_w64(want);
     if (wmask != 0ull) {
       const int n = __builtin_popcountll(wmask);
@@ -125,12 +128,21 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
           }
         }
       }
-    } else if (mode == MODE_SSA) {
-      mode = eng.template step<true>(jr, cnt, mode);
-      if (mode == MODE_DONE) {
-        int32_t* fl = jr.p().flags;
-        if (fl) fl[sim] = eng.flag;
-        cnt.flagged += (eng.flag != 0);
+    } else {
+      // no lane decides (pure SSA, or every live lane inside a burst): up to
+      // 1 + ssa_slice SSA steps before the next refill
+      const int slice = jr.p().ssa_slice;
+      for (int s = 0; s <= slice; ++s) {
+        if (!wave_any(mode == MODE_SSA)) break;
+        if (s > 0) jr.refresh();
+        if (mode == MODE_SSA) {
+          mode = eng.template step<true>(jr, cnt, mode);
+          if (mode == MODE_DONE) {
+            int32_t* fl = jr.p().flags;
+            if (fl) fl[sim] = eng.flag;
+            cnt.flagged += (eng.flag != 0);
+          }
+        }
       }
     }
   }
