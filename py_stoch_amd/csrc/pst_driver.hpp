@@ -19,7 +19,8 @@
 // steps for them: a wavefront lasts as long as its slowest simulation, and a
 // burst taken one step per pass would add 100 passes to that simulation's
 // path (8 steps per pass: Lotka-Volterra 81 -> 70 ms, Schlogl 110 -> 76 ms).
-// A pass in which no lane decides runs up to 1 + ssa_slice SSA steps.
+// A pass in which no lane decides runs up to 1 + ssa_slice SSA steps (2 +
+// 2 ssa_slice for pure SSA: 68 -> 66 ms on Schlogl, 65,536 simulations).
 // Every simulation executes exactly the reference's sequence
 // (decision, burst of ssa_burst SSA steps, decision, ...); only the
 // interleaving of different simulations within a wavefront changes.
@@ -130,8 +131,9 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
       }
     } else {
       // no lane decides (pure SSA, or every live lane inside a burst): up to
-      // 1 + ssa_slice SSA steps before the next refill
-      const int slice = jr.p().ssa_slice;
+      // 1 + ssa_slice SSA steps before the next refill, 16 for pure SSA where
+      // no lane waits to decide
+      const int slice = jr.p().method == 2 ? 2 * jr.p().ssa_slice + 1 : jr.p().ssa_slice;
       for (int s = 0; s <= slice; ++s) {
         if (!wave_any(mode == MODE_SSA)) break;
         if (s > 0) jr.refresh();
