@@ -343,15 +343,33 @@ struct EngineF32 {
       const double target = (double)u24(cw1) * asum;
       double cum = 0.0;
       int last = -1;
+      if constexpr (SSA_ONLY) {
+        // The running sum never decreases and rho2 * asum > 0, so that j is
+        // the number of positions whose sum is still below the target (a
+        // masked-out position repeats its predecessor's sum and is never the
+        // first to reach it): no branches in the SSA-only steps (Schlogl pure
+        // SSA 68 -> 59 ms; the leap step keeps the loop, which measured faster
+        // there).  The last positive masked reaction covers a sum that ends
+        // below the target.
+        int below = 0;
 #pragma unroll
-      for (int j = 0; j < MP; ++j) {
-        if (msk[j] && sel < 0) {
-          if (a[j] > 0.0f) last = j;
-          cum += (double)a[j];
-          if (cum >= target) sel = j;
+        for (int j = 0; j < MP; ++j) {
+          cum += msk[j] ? (double)a[j] : 0.0;
+          last = (msk[j] && a[j] > 0.0f) ? j : last;
+          below += (cum < target) ? 1 : 0;
         }
+        sel = below < MP ? below : last;
+      } else {
+#pragma unroll
+        for (int j = 0; j < MP; ++j) {
+          if (msk[j] && sel < 0) {
+            if (a[j] > 0.0f) last = j;
+            cum += (double)a[j];
+            if (cum >= target) sel = j;
+          }
+        }
+        if (sel < 0) sel = last;
       }
-      if (sel < 0) sel = last;
     }
     int32_t xn[NP];
     float tau = tau_f;
