@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define PST_ABI_VERSION 2
+#define PST_ABI_VERSION 3
 
 /* status codes */
 #define PST_OK 0
@@ -131,6 +131,7 @@ typedef struct pst_stats {
   int64_t inversion_steps;    /* Poisson inversion search steps               */
   int64_t leap_decisions;     /* tau-leap decisions taken (P1_P2 bodies)      */
   int64_t sample_rows;        /* sample rows written (per recorded row of O)  */
+  int64_t flagged_events;     /* the part of events from flagged simulations  */
   double kernel_ms;           /* device time of the simulation kernel         */
 } pst_stats;
 

@@ -70,6 +70,7 @@ class OrcStats(ctypes.Structure):
         ("rejected_leaps", ctypes.c_int64),
         ("ssa_steps", ctypes.c_int64),
         ("flagged", ctypes.c_int64),
+        ("flagged_events", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -307,6 +307,7 @@ typedef struct {
     int64_t rejected_leaps;
     int64_t ssa_steps;
     int64_t flagged;
+    int64_t flagged_events;  /* events of the simulations counted in flagged */
 } orc_stats;
 
 #define ORC_NMAX 64
@@ -650,25 +651,26 @@ int orc_simulate(const orc_network* net, const double* rates, const int32_t* x0,
     if (net->n_reactions < 1 || net->n_reactions > ORC_MMAX) return ORC_EINVAL;
     if (c->ita < 1 || c->kappa < 1 || c->n_sims < 0 || o_stride < c->n_sims) return ORC_EINVAL;
     if (c->sims_per_set < 1) return ORC_EINVAL;
-    int64_t ev = 0, lp = 0, rj = 0, ss = 0, fl = 0;
+    int64_t ev = 0, lp = 0, rj = 0, ss = 0, fl = 0, fe = 0;
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #else
     (void)threads;
 #endif
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : ev, lp, rj, ss, fl)
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : ev, lp, rj, ss, fl, fe)
     for (int64_t s = 0; s < c->n_sims; ++s) {
-        orc_stats st = {0, 0, 0, 0, 0};
+        orc_stats st = {0, 0, 0, 0, 0, 0};
         int64_t set = s / c->sims_per_set;
         int f = simulate_one(net, rates + set * net->n_reactions, x0 + set * net->n_species,
                              c, o_stride, s, O, &st);
         if (flags) flags[s] = f;
         ev += st.events; lp += st.leaps; rj += st.rejected_leaps; ss += st.ssa_steps;
         fl += (f != 0);
+        fe += (f != 0) ? st.events : 0;
     }
     if (stats) {
         stats->events = ev; stats->leaps = lp; stats->rejected_leaps = rj;
-        stats->ssa_steps = ss; stats->flagged = fl;
+        stats->ssa_steps = ss; stats->flagged = fl; stats->flagged_events = fe;
     }
     return ORC_OK;
 }

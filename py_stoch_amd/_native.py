@@ -12,6 +12,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libpst_hip.so")
+ABI_VERSION = 3  # PST_ABI_VERSION of include/pst.h
 
 METHOD_TAU_LEAPING = 0
 METHOD_FORCED_SSA = 1
@@ -84,6 +85,7 @@ class PstStats(ctypes.Structure):
         ("inversion_steps", ctypes.c_int64),
         ("leap_decisions", ctypes.c_int64),
         ("sample_rows", ctypes.c_int64),
+        ("flagged_events", ctypes.c_int64),
         ("kernel_ms", ctypes.c_double),
     ]
 
@@ -169,6 +171,9 @@ def load():
     L.pst_last_error.restype = ctypes.c_char_p
     L.pst_abi_version.argtypes = []
     L.pst_abi_version.restype = ctypes.c_int
+    if L.pst_abi_version() != ABI_VERSION:
+        raise NativeError("%s implements C ABI %d, this binding expects %d: rebuild it" % (
+            LIB_PATH, L.pst_abi_version(), ABI_VERSION))
     _lib = L
     return L
 

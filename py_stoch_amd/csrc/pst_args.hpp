@@ -28,7 +28,7 @@ struct NetTab {
 
 enum StatIdx {
   ST_EVENTS = 0, ST_LEAPS, ST_REJECTED, ST_SSA, ST_FLAGGED,
-  ST_UNIFORMS, ST_PTRS, ST_INV, ST_DECISIONS, ST_SAMPLES,
+  ST_UNIFORMS, ST_PTRS, ST_INV, ST_DECISIONS, ST_SAMPLES, ST_FEVENTS,
   ST_COUNT = 16
 };
 

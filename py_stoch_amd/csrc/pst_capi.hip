@@ -660,6 +660,7 @@ int pst_plan_read_stats(pst_plan* p, pst_stats* st) {
   st->inversion_steps = (int64_t)h[pst::ST_INV];
   st->leap_decisions = (int64_t)h[pst::ST_DECISIONS];
   st->sample_rows = (int64_t)h[pst::ST_SAMPLES];
+  st->flagged_events = (int64_t)h[pst::ST_FEVENTS];
   st->kernel_ms = (double)ms;
   return PST_OK;
 }

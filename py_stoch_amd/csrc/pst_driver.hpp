@@ -34,6 +34,7 @@ enum Mode { MODE_DECIDE = 0, MODE_SSA = 1, MODE_DONE = 2 };
 
 struct Counters {
   unsigned long long events;
+  unsigned long long fevents;  // events of simulations that ended flagged
   unsigned int leaps, rejected, ssa, flagged;
   Work w;
 };
@@ -69,6 +70,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
   Engine eng;
   int mode = MODE_DONE;
   int64_t sim = -1;
+  unsigned long long ev_start = 0;  // cnt.events when this lane's simulation began
   int job = 0;
   bool exhausted = false;
 
@@ -92,6 +94,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
         const int64_t s = (int64_t)base + rank;
         if (s < n_sims) {
           sim = s;
+          ev_start = cnt.events;
           mode = eng.start(jr, s);
         }
       }
@@ -113,6 +116,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
           int32_t* fl = jr.p().flags;
           if (fl) fl[sim] = eng.flag;
           cnt.flagged += (eng.flag != 0);
+          cnt.fevents += (eng.flag != 0) ? cnt.events - ev_start : 0ull;
         }
       }
       // SSA-only steps while lanes are inside a burst
@@ -126,6 +130,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
             int32_t* fl = jr.p().flags;
             if (fl) fl[sim] = eng.flag;
             cnt.flagged += (eng.flag != 0);
+            cnt.fevents += (eng.flag != 0) ? cnt.events - ev_start : 0ull;
           }
         }
       }
@@ -143,18 +148,19 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
             int32_t* fl = jr.p().flags;
             if (fl) fl[sim] = eng.flag;
             cnt.flagged += (eng.flag != 0);
+            cnt.fevents += (eng.flag != 0) ? cnt.events - ev_start : 0ull;
           }
         }
       }
     }
   }
 
-  const unsigned long long vals[ST_SAMPLES + 1] = {cnt.events, cnt.leaps, cnt.rejected, cnt.ssa, cnt.flagged,
+  const unsigned long long vals[ST_FEVENTS + 1] = {cnt.events, cnt.leaps, cnt.rejected, cnt.ssa, cnt.flagged,
                                                    cnt.w.uniforms, cnt.w.ptrs, cnt.w.inv, cnt.w.decisions,
-                                                   cnt.w.samples};
+                                                   cnt.w.samples, cnt.fevents};
   // (32-bit per-lane counters: a lane's share of one launch stays far below 2^32)
 #pragma unroll
-  for (int i = 0; i <= ST_SAMPLES; ++i) {
+  for (int i = 0; i <= ST_FEVENTS; ++i) {
     const unsigned long long sum = wave_sum(vals[i]);
     if (lane == 0) atomicAdd(&stats[i], sum);
   }

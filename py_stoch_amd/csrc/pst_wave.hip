@@ -26,20 +26,23 @@ __global__ void __launch_bounds__(256) sim_wave_kernel(const WaveJob* __restrict
       if (lane == 0) s = atomicAdd(&queue[job], 1u);
       s = __shfl(s, 0, 64);
       if ((int64_t)s >= (int64_t)J.p.n_sims) break;
+      const unsigned long long ev_start = cnt.events;
       eng.start(J, L, lane, (int64_t)s);
       while (!eng.step(J, L, lane, cnt)) {
       }
+      const int fl = __shfl(eng.flag, 0, 64);
+      cnt.fevents += (fl != 0) ? cnt.events - ev_start : 0ull;  // summed over lanes below
       if (lane == 0) {
         if (J.p.flags) J.p.flags[s] = eng.flag;
         cnt.flagged += (eng.flag != 0);
       }
     }
   }
-  const unsigned long long vals[ST_SAMPLES + 1] = {cnt.events, cnt.leaps, cnt.rejected, cnt.ssa, cnt.flagged,
+  const unsigned long long vals[ST_FEVENTS + 1] = {cnt.events, cnt.leaps, cnt.rejected, cnt.ssa, cnt.flagged,
                                                    cnt.w.uniforms, cnt.w.ptrs, cnt.w.inv, cnt.w.decisions,
-                                                   cnt.w.samples};
+                                                   cnt.w.samples, cnt.fevents};
 #pragma unroll
-  for (int i = 0; i <= ST_SAMPLES; ++i) {
+  for (int i = 0; i <= ST_FEVENTS; ++i) {
     const unsigned long long sum = wave_sum(vals[i]);
     if (lane == 0) atomicAdd(&stats[i], sum);
   }

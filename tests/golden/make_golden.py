@@ -13,7 +13,7 @@
                            t = 1, 2, 5, 10
   oracle_trajectories.npz  FP64 oracle trajectories (oracle/pst_oracle.c) of
                            three small seeded workloads, with the oracle's
-                           event / leap / SSA-step counts:
+                           event / leap / SSA-step / flag counts:
                              bd  birth-death 0 -> X (10), X -> 0 (0.1), X0 = 0,
                                  pure SSA, 256 simulations, t_end = 100,
                                  101 samples, seed 0
@@ -83,7 +83,7 @@ def oracle_runs():
         out[name + "_O"] = O
         out[name + "_flags"] = flags
         out[name + "_stats"] = np.array([st["events"], st["leaps"], st["rejected_leaps"], st["ssa_steps"],
-                                         st["flagged"]], np.int64)
+                                         st["flagged"], st["flagged_events"]], np.int64)
         out[name + "_settings"] = np.array([method, t_max, ita, U, seed], np.float64)
     return out
 

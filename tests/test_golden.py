@@ -42,7 +42,7 @@ def test_oracle_reproduces_committed_trajectories(oracle_mod, name):
                                      E=a.E, n_sims=a.U, seed=seed)
     assert np.array_equal(O2, O)
     assert np.array_equal(f2, flags)
-    assert [st[k] for k in ("events", "leaps", "rejected_leaps", "ssa_steps", "flagged")] == stats.tolist()
+    assert [st[k] for k in ("events", "leaps", "rejected_leaps", "ssa_steps", "flagged", "flagged_events")] == stats.tolist()
 
 
 def test_golden_trajectories_obey_the_models():

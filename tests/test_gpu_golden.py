@@ -26,7 +26,7 @@ def test_fp64_kernels_reproduce_golden_trajectories(native, name, specialize):
     assert mism.shape[0] == 0, "first mismatch %s gpu=%d golden=%d" % (
         mism[0].tolist(), O_g[tuple(mism[0])], O[tuple(mism[0])])
     assert np.array_equal(f_g, flags)
-    assert [st[k] for k in ("events", "leaps", "rejected_leaps", "ssa_steps", "flagged")] == stats.tolist()
+    assert [st[k] for k in ("events", "leaps", "rejected_leaps", "ssa_steps", "flagged", "flagged_events")] == stats.tolist()
 
 
 @pytest.mark.parametrize("method", [0, 2])

@@ -22,7 +22,7 @@ def _compare(native, oracle_mod, spn, E, t_max, ita, U, method, seed=7, precisio
     assert mism.shape[0] == 0, "first mismatch %s gpu=%d cpu=%d" % (
         mism[0].tolist(), O_g[tuple(mism[0])], O_c[tuple(mism[0])])
     assert np.array_equal(f_g, f_c)
-    for k in ("events", "leaps", "rejected_leaps", "ssa_steps", "flagged"):
+    for k in ("events", "leaps", "rejected_leaps", "ssa_steps", "flagged", "flagged_events"):
         assert s_g[k] == s_c[k], (k, s_g[k], s_c[k])
     return O_g, s_g
 
@@ -128,5 +128,5 @@ def test_step_budget_bit_exact(native, oracle_mod, specialize):
         O_c, f_c, s_c = oracle_mod.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, **kw)
         assert np.array_equal(f_g, f_c) and (f_c & native.FLAG_STEPS).any()
         assert np.array_equal(O_g, O_c)
-        for k in ("events", "leaps", "ssa_steps", "flagged"):
+        for k in ("events", "leaps", "ssa_steps", "flagged", "flagged_events"):
             assert s_g[k] == s_c[k], k

@@ -23,7 +23,7 @@ def _compare(native, oracle_mod, spn, E, t_max, ita, U, method, seed=7):
     assert mism.shape[0] == 0, "first mismatch %s gpu=%d cpu=%d" % (
         mism[0].tolist(), O_g[tuple(mism[0])], O_c[tuple(mism[0])])
     assert np.array_equal(f_g, f_c)
-    for k in ("events", "leaps", "rejected_leaps", "ssa_steps", "flagged"):
+    for k in ("events", "leaps", "rejected_leaps", "ssa_steps", "flagged", "flagged_events"):
         assert s_g[k] == s_c[k], (k, s_g[k], s_c[k])
     return O_g, s_g
 

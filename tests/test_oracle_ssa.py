@@ -178,3 +178,22 @@ def test_step_budget_ends_simulations(oracle_mod):
         f = nz[-1] + 1
         assert 1 <= f < O.shape[1] and (O[:, f:, s] == 0).all()
         assert np.array_equal(O[:, :f, s], O2[:, :f, s])
+
+
+def test_flagged_events_are_the_events_of_flagged_simulations(oracle_mod):
+    """stats.flagged_events is the sum of the event counts of exactly the
+    simulations that end flagged: rerun each simulation alone (same stream,
+    through sim_offset) and add up the flagged ones."""
+    a = models.make_args(models.lotka_volterra(), E=[0, 1], t_max=2.0, ita=21, U=48)
+    tabs, rates = tables_of(a), a.rate_fn(a.c)
+    kw = dict(t_max=2.0, I=a.I, E=a.E, seed=9, max_steps=300)
+    _, F, st = oracle_mod.simulate(tabs, rates, a.x_0, n_sims=48, **kw)
+    assert 0 < st["flagged"] < 48  # some simulations run out of steps, some do not
+    total = flagged = 0
+    for s in range(48):
+        _, f1, s1 = oracle_mod.simulate(tabs, rates, a.x_0, n_sims=1, sim_offset=s, **kw)
+        assert f1[0] == F[s]
+        total += s1["events"]
+        flagged += s1["events"] if f1[0] != 0 else 0
+        assert s1["flagged_events"] == (s1["events"] if f1[0] != 0 else 0)
+    assert total == st["events"] and flagged == st["flagged_events"] > 0
