@@ -299,6 +299,12 @@ def other_workload(args):
     print("bench: %s, compiling kernels" % wl, file=sys.stderr, flush=True)
     p_mix = _native.BatchPlan(workloads.jobs(specs, _native.PRECISION_MIXED))
     p_f64 = _native.BatchPlan(workloads.jobs(specs, _native.PRECISION_FP64))
+    tune = None
+    if not args.no_tune:
+        print("bench: %s, autotuning launch shapes" % wl, file=sys.stderr, flush=True)
+        ptrs, strides = [O.data_ptr() for O in O_list], [s.n_sims for s in specs]
+        tune = {"mixed": p_mix.autotune(ptrs, strides, None, stream.cuda_stream, fraction=0.25),
+                "fp64": p_f64.autotune(ptrs, strides, None, stream.cuda_stream, fraction=0.25)}
     print("bench: %s, mixed" % wl, file=sys.stderr, flush=True)
     wall_m, kms_m, st_m = _time_batch(p_mix, O_list, args.steps, args.warmup, stream)
     O_mix = [O.clone() for O in O_list]
@@ -359,6 +365,7 @@ def other_workload(args):
         "speedup_vs_fp64_specialized": (wall_f / k_f) / (wall_m / args.steps),
         "events_per_s_ratio_vs_fp64_specialized": value / fp64_value,
         "counters": {k: v for k, v in st_m.items() if k != "kernel_ms"}, "accuracy_vs_fp64": acc,
+        "launch_autotune": tune,
     }
 
 
@@ -433,6 +440,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the library's default launch shapes (skip the launch autotuner)")
     ap.add_argument("--acc-sets", type=int, default=16,
                     help="independent sets of 65536 simulations per path for the accuracy check")
     ap.add_argument("--workload", default="lotka-volterra",
@@ -489,6 +498,14 @@ def main():
         print("bench: traffic probe done (copy %d bytes, one mixed launch)" % COPY_BYTES, file=sys.stderr)
         return
 
+    # launch shapes (simulations per wavefront, SSA steps per pass) of every
+    # plan, timed on the first quarter of the samples; results do not depend on them
+    tune = None
+    if not args.no_tune:
+        print("bench: autotuning launch shapes", file=sys.stderr, flush=True)
+        tune = {name: plan.autotune(O.data_ptr(), U, None, stream.cuda_stream, fraction=0.25)
+                for name, plan, O in (("mixed", p_mixed, O_m), ("fp64_plain", p_plain, O_f),
+                                      ("fp64_specialized", p_fp64, O_f))}
     print("bench: mixed precision, %d warmup + %d timed steps" % (args.warmup, args.steps),
           file=sys.stderr, flush=True)
     wall_m, kern_ms_m, st_m = time_plan(p_mixed, O_m, args.steps, args.warmup, stream)
@@ -551,6 +568,7 @@ def main():
         "speedup_vs_fp64_specialized": (wall_s / k_f) / (wall_m / args.steps),
         "events_per_s_ratio_vs_fp64": value / fp64_value,
         "accuracy_vs_fp64": acc,
+        "launch_autotune": tune,
         "counters": {k: v for k, v in st_m.items() if k != "kernel_ms"},
         "roofline": roof,
         "cpu_baseline": cpu,

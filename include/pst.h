@@ -206,6 +206,29 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** plan);
 int pst_batch_plan_execute(pst_plan* plan, int32_t* const* O_dev, const int64_t* o_strides,
                            int32_t* const* flags_dev, void* stream);
 
+/*
+ * Launch autotuner.  Times candidate launch shapes of a plan -- simulations
+ * per wavefront (the library's choice, and half of it) x SSA-only steps per
+ * pass while lanes sit in a burst (3, 7, 15) -- on the first `fraction` of
+ * every job's sample times (0 < fraction <= 1; writes into the given
+ * buffers, which need the shapes of pst_batch_plan_execute), and keeps the
+ * fastest for the plan's later executions (another shape replaces the
+ * default only if it is at least 2% faster).  Waits for the runs.  It never
+ * touches the method's settings (epsilon, n_c, burst length), and no result
+ * bit depends on the shape chosen (the stream is keyed by simulation id and
+ * a pass only interleaves simulations).  No reference counterpart: the
+ * reference picks its block size from fixed thresholds on U
+ * (cuTauLeaping.py:162-214).
+ */
+typedef struct pst_launch {
+  int32_t lanes_shift;        /* simulations per wavefront = library choice >> lanes_shift */
+  int32_t ssa_slice;          /* SSA-only steps per pass (pure SSA: 2 ssa_slice + 1) */
+  double kernel_ms;           /* kernel time of the chosen shape on the tuning horizon */
+  double default_ms;          /* same for the library's default shape (0, 7) */
+} pst_launch;
+int pst_plan_autotune(pst_plan* plan, int32_t* const* O_dev, const int64_t* o_strides,
+                      int32_t* const* flags_dev, void* stream, double fraction, pst_launch* chosen);
+
 /* largest network the compiled kernels accept (species, reactions) */
 void pst_max_network(int32_t* max_species, int32_t* max_reactions);
 

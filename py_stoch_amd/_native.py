@@ -95,6 +95,19 @@ class PstStats(ctypes.Structure):
         return d
 
 
+class PstLaunch(ctypes.Structure):
+    _fields_ = [
+        ("lanes_shift", ctypes.c_int32),
+        ("ssa_slice", ctypes.c_int32),
+        ("kernel_ms", ctypes.c_double),
+        ("default_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {"lanes_shift": int(self.lanes_shift), "ssa_slice": int(self.ssa_slice),
+                "kernel_ms": float(self.kernel_ms), "default_ms": float(self.default_ms)}
+
+
 class PstJob(ctypes.Structure):
     _fields_ = [
         ("net", PstNetwork),
@@ -119,6 +132,7 @@ EXPORTED_SYMBOLS = (
     "pst_max_network",
     "pst_last_error",
     "pst_abi_version",
+    "pst_plan_autotune",
 )
 
 _lib = None
@@ -153,6 +167,10 @@ def load():
                                          ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_void_p),
                                          ctypes.c_void_p]
     L.pst_batch_plan_execute.restype = ctypes.c_int
+    L.pst_plan_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                    ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_void_p),
+                                    ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(PstLaunch)]
+    L.pst_plan_autotune.restype = ctypes.c_int
     L.pst_plan_create.argtypes = [ctypes.POINTER(PstNetwork), ctypes.POINTER(PstConfig), _P_F64,
                                   _P_I32, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
     L.pst_plan_create.restype = ctypes.c_int
@@ -251,6 +269,17 @@ def simulate(tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_F
     return O, (flags[: int(n_sims)] if flags is not None else None), st.as_dict()
 
 
+def _autotune(h, O_ptrs, strides, flag_ptrs, stream_ptr, fraction):
+    n = len(O_ptrs)
+    Op = (ctypes.c_void_p * n)(*[int(p) for p in O_ptrs])
+    Sp = (ctypes.c_int64 * n)(*[int(s) for s in strides])
+    Fp = (ctypes.c_void_p * n)(*[int(p) if p else None for p in flag_ptrs]) if flag_ptrs else None
+    out = PstLaunch()
+    _check(load().pst_plan_autotune(h, Op, Sp, Fp, ctypes.c_void_p(int(stream_ptr)) if stream_ptr else None,
+                                    float(fraction), ctypes.byref(out)), "pst_plan_autotune")
+    return out.as_dict()
+
+
 class Plan:
     """Device-resident run: tables uploaded once, trajectories left in device memory."""
 
@@ -280,6 +309,12 @@ class Plan:
     def set_stream(self, seed, sim_offset=0):
         """Re-key the random stream of the following executions."""
         _check(load().pst_plan_set_stream(self._h, int(seed), int(sim_offset)), "pst_plan_set_stream")
+
+    def autotune(self, O_dev_ptr, o_stride, flags_dev_ptr=None, stream_ptr=None, fraction=0.25):
+        """Time the candidate launch shapes on the first `fraction` of the
+        sample times (writing into O_dev_ptr) and keep the fastest; returns it."""
+        return _autotune(self._h, [O_dev_ptr], [o_stride], [flags_dev_ptr] if flags_dev_ptr else None,
+                         stream_ptr, fraction)
 
     def stats(self):
         st = PstStats()
@@ -356,6 +391,11 @@ class BatchPlan:
         _check(load().pst_batch_plan_execute(self._h, Op, Sp, Fp,
                                              ctypes.c_void_p(int(stream_ptr)) if stream_ptr else None),
                "pst_batch_plan_execute")
+
+    def autotune(self, O_ptrs, strides, flag_ptrs=None, stream_ptr=None, fraction=0.25):
+        """Time the candidate launch shapes on the first `fraction` of every
+        job's sample times and keep the fastest; returns it."""
+        return _autotune(self._h, O_ptrs, strides, flag_ptrs, stream_ptr, fraction)
 
     def set_stream(self, seed, sim_offset=0):
         _check(load().pst_plan_set_stream(self._h, int(seed), int(sim_offset)), "pst_plan_set_stream")

@@ -8,6 +8,7 @@
 // trajectories (and 16 counters) come back.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdarg>
@@ -314,6 +315,8 @@ struct pst_plan {
   unsigned long long* d_stats = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool executed = false;
+  int lanes_shift = 0;        // launch shape (pst_plan_autotune)
+  int ssa_slice = kSsaSlice;
 };
 
 extern "C" {
@@ -501,19 +504,20 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
   }
   hipStream_t s = (hipStream_t)stream;
   hipError_t e;
-  // simulations per wavefront, per launch
+  // simulations per wavefront, per launch, and SSA-only steps per pass
   const int simds = p->cu_count * 4;
   for (int c = 0; c < pst::NUM_CLASSES; ++c) {
     std::vector<int> ids;
     for (int q = 0; q < n_jobs; ++q)
       if (p->jobs[q].cls == c) ids.push_back(q);
-    const int L = lanes_for(p->jobs, ids, simds);
+    const int L = std::max(1, lanes_for(p->jobs, ids, simds) >> p->lanes_shift);
     for (int q : ids) p->jobs[q].p.lanes = L;
   }
   for (FixedGroup& g : p->fixed) {
-    const int L = lanes_for(p->jobs, g.jobs, simds);
+    const int L = std::max(1, lanes_for(p->jobs, g.jobs, simds) >> p->lanes_shift);
     for (int q : g.jobs) p->jobs[q].p.lanes = L;
   }
+  for (PlanJob& pj : p->jobs) pj.p.ssa_slice = p->ssa_slice;
   // job descriptors, one device array per size class
   int count[pst::NUM_CLASSES] = {0, 0, 0};
   for (int c = 0; c < pst::NUM_CLASSES; ++c) {
@@ -662,6 +666,66 @@ int pst_plan_read_stats(pst_plan* p, pst_stats* st) {
   st->sample_rows = (int64_t)h[pst::ST_SAMPLES];
   st->flagged_events = (int64_t)h[pst::ST_FEVENTS];
   st->kernel_ms = (double)ms;
+  return PST_OK;
+}
+
+int pst_plan_autotune(pst_plan* p, int32_t* const* O_dev, const int64_t* o_strides, int32_t* const* flags_dev,
+                      void* stream, double fraction, pst_launch* chosen) {
+  g_err[0] = 0;
+  if (!p || !chosen) return fail(PST_EINVAL, "null plan or result");
+  if (!(fraction > 0.0 && fraction <= 1.0)) return fail(PST_EINVAL, "fraction %g outside (0, 1]", fraction);
+  // the tuning horizon: the first ceil(fraction * ita) sample times of every job
+  std::vector<int32_t> ita(p->jobs.size());
+  for (size_t q = 0; q < p->jobs.size(); ++q) {
+    ita[q] = p->jobs[q].p.ita;
+    p->jobs[q].p.ita = std::max(1, std::min(ita[q], (int32_t)std::ceil(fraction * (double)ita[q])));
+  }
+  const int old_shift = p->lanes_shift, old_slice = p->ssa_slice;
+  auto restore = [&]() {
+    for (size_t q = 0; q < p->jobs.size(); ++q) p->jobs[q].p.ita = ita[q];
+  };
+  auto timed = [&](int shift, int slice, double* ms) -> int {
+    p->lanes_shift = shift;
+    p->ssa_slice = slice;
+    const int rc = pst_batch_plan_execute(p, O_dev, o_strides, flags_dev, stream);
+    if (rc != PST_OK) return rc;
+    hipError_t e = hipEventSynchronize(p->ev1);
+    if (e != hipSuccess) return hip_fail(e, "simulation kernel");
+    float f = 0.0f;
+    if ((e = hipEventElapsedTime(&f, p->ev0, p->ev1)) != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+    *ms = (double)f;
+    return PST_OK;
+  };
+  double ms = 0.0, def_ms = 0.0, best_ms = 0.0;
+  int best_shift = 0, best_slice = kSsaSlice;
+  int rc = timed(0, kSsaSlice, &ms);  // warm-up (first launch of each kernel)
+  if (rc == PST_OK) rc = timed(0, kSsaSlice, &def_ms);
+  best_ms = def_ms;
+  const int slices[3] = {3, 7, 15};
+  for (int shift = 0; shift <= 1 && rc == PST_OK; ++shift) {
+    for (int k = 0; k < 3 && rc == PST_OK; ++k) {
+      if (shift == 0 && slices[k] == kSsaSlice) continue;  // the default, timed above
+      rc = timed(shift, slices[k], &ms);
+      // another shape must beat the default by 2 % (run-to-run noise)
+      if (rc == PST_OK && ms < best_ms && ms < 0.98 * def_ms) {
+        best_ms = ms;
+        best_shift = shift;
+        best_slice = slices[k];
+      }
+    }
+  }
+  restore();
+  if (rc != PST_OK) {
+    p->lanes_shift = old_shift;
+    p->ssa_slice = old_slice;
+    return rc;
+  }
+  p->lanes_shift = best_shift;
+  p->ssa_slice = best_slice;
+  chosen->lanes_shift = best_shift;
+  chosen->ssa_slice = best_slice;
+  chosen->kernel_ms = best_ms;
+  chosen->default_ms = def_ms;
   return PST_OK;
 }
 

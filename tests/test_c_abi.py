@@ -93,3 +93,12 @@ def test_pure_ssa_entry_rejected_on_tau_leaping_call(native):
                                      m.x0.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), 1,
                                      O.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), None, None)
         native._check(rc, "pst_tau_leaping_run")
+
+
+def test_autotune_argument_checks(native):
+    """pst_plan_autotune rejects a null plan or result and a fraction outside
+    (0, 1] before touching a device."""
+    L = native.load()
+    out = native.PstLaunch()
+    assert L.pst_plan_autotune(None, None, None, None, None, 0.5, ctypes.byref(out)) == -1  # PST_EINVAL
+    assert b"null" in L.pst_last_error()

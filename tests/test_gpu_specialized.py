@@ -161,3 +161,56 @@ def test_immigration_birth_death_mean(native, precision):
     mean = O[0].mean(axis=-1)
     se = np.sqrt(np.maximum(O[0].var(axis=-1), 1.0) / O.shape[-1])
     assert np.all(np.abs(mean - m) <= 4 * se + 0.02 * m)
+
+
+class _DeviceArray:
+    """int32 device buffer from the HIP runtime the library links against
+    (libamdhip64.so.7, already loaded by it, so one runtime instance)."""
+
+    def __init__(self, shape):
+        import ctypes
+        self.hip = ctypes.CDLL("libamdhip64.so.7")
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * 4
+        self.ptr = ctypes.c_void_p()
+        assert self.hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(self.nbytes)) == 0
+
+    def get(self):
+        import ctypes
+        out = np.empty(self.shape, np.int32)
+        assert self.hip.hipDeviceSynchronize() == 0
+        assert self.hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, ctypes.c_size_t(self.nbytes), 2) == 0
+        return out
+
+    def free(self):
+        self.hip.hipFree(self.ptr)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_autotuned_plan_gives_the_same_results(native, oracle_mod, precision):
+    """The autotuner changes only the launch shape: after it, the FP64 plan is
+    still bit-exact against the oracle and the mixed plan reproduces an
+    untuned plan's trajectories."""
+    a = models.make_args(models.lotka_volterra(), E=[0, 1], t_max=4.0, ita=41, U=8192)
+    kw = dict(method=0, t_max=4.0, I=a.I, E=a.E, n_sims=8192, seed=12, precision=precision, specialize=True)
+    tabs, rates = tables_of(a), a.rate_fn(a.c)
+    print("case: autotune precision=%d" % precision, flush=True)
+    O1, O2 = _DeviceArray((2, 41, 8192)), _DeviceArray((2, 41, 8192))
+    try:
+        p1 = native.Plan(tabs, rates, a.x_0, **kw)
+        p2 = native.Plan(tabs, rates, a.x_0, **kw)
+        ch = p1.autotune(O1.ptr.value, 8192, fraction=0.5)
+        assert ch["lanes_shift"] in (0, 1) and ch["ssa_slice"] in (3, 7, 15)
+        assert 0.0 < ch["kernel_ms"] <= ch["default_ms"]
+        p1.execute(O1.ptr.value, 8192)
+        p2.execute(O2.ptr.value, 8192)
+        s1, s2 = p1.stats(), p2.stats()
+        g1, g2 = O1.get(), O2.get()
+    finally:
+        O1.free()
+        O2.free()
+    assert np.array_equal(g1, g2)
+    assert all(s1[k] == s2[k] for k in ("events", "leaps", "ssa_steps", "flagged", "flagged_events"))
+    if precision == 0:
+        Oc, _, sc = oracle_mod.simulate(tabs, rates, a.x_0, method=0, t_max=4.0, I=a.I, E=a.E, n_sims=8192, seed=12)
+        assert np.array_equal(g1, Oc) and s1["events"] == sc["events"]
