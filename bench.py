@@ -303,8 +303,8 @@ def other_workload(args):
     if not args.no_tune:
         print("bench: %s, autotuning launch shapes" % wl, file=sys.stderr, flush=True)
         ptrs, strides = [O.data_ptr() for O in O_list], [s.n_sims for s in specs]
-        tune = {"mixed": p_mix.autotune(ptrs, strides, None, stream.cuda_stream, fraction=0.25),
-                "fp64": p_f64.autotune(ptrs, strides, None, stream.cuda_stream, fraction=0.25)}
+        tune = {"mixed": p_mix.autotune(ptrs, strides, None, stream.cuda_stream, fraction=0.5),
+                "fp64": p_f64.autotune(ptrs, strides, None, stream.cuda_stream, fraction=0.5)}
     print("bench: %s, mixed" % wl, file=sys.stderr, flush=True)
     wall_m, kms_m, st_m = _time_batch(p_mix, O_list, args.steps, args.warmup, stream)
     O_mix = [O.clone() for O in O_list]
@@ -499,11 +499,11 @@ def main():
         return
 
     # launch shapes (simulations per wavefront, SSA steps per pass) of every
-    # plan, timed on the first quarter of the samples; results do not depend on them
+    # plan, timed on the first half of the samples; results do not depend on them
     tune = None
     if not args.no_tune:
         print("bench: autotuning launch shapes", file=sys.stderr, flush=True)
-        tune = {name: plan.autotune(O.data_ptr(), U, None, stream.cuda_stream, fraction=0.25)
+        tune = {name: plan.autotune(O.data_ptr(), U, None, stream.cuda_stream, fraction=0.5)
                 for name, plan, O in (("mixed", p_mixed, O_m), ("fp64_plain", p_plain, O_f),
                                       ("fp64_specialized", p_fp64, O_f))}
     print("bench: mixed precision, %d warmup + %d timed steps" % (args.warmup, args.steps),
