@@ -15,9 +15,8 @@ namespace pst {
 // instead of 227 VGPRs at two, which hides more latency than the spills cost
 // (first 48 random networks: 1,508 -> 1,264 ms; three: 1,339 ms; five: 1,622 ms).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
-sim_wave_kernel(const WaveJob* __restrict__ jobs, int n_jobs,
-                                                       unsigned int* __restrict__ queue,
-                                                       unsigned long long* __restrict__ stats) {
+sim_wave_kernel(const WaveJob* __restrict__ jobs, int n_jobs, unsigned int* __restrict__ queue,
+                unsigned long long* __restrict__ stats) {
   __shared__ WaveLds lds[4];
   const int lane = (int)(threadIdx.x & 63u);
   WaveLds& L = lds[threadIdx.x >> 6];
