@@ -136,8 +136,8 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
       }
     } else {
       // no lane decides (pure SSA, or every live lane inside a burst): up to
-      // 1 + ssa_slice SSA steps before the next refill, 16 for pure SSA where
-      // no lane waits to decide
+      // 1 + ssa_slice SSA steps before the next refill, 2 + 2 ssa_slice for
+      // pure SSA, where no lane waits to decide
       const int slice = jr.p().method == 2 ? 2 * jr.p().ssa_slice + 1 : jr.p().ssa_slice;
       for (int s = 0; s <= slice; ++s) {
         if (!wave_any(mode == MODE_SSA)) break;
