@@ -555,28 +555,54 @@ struct EngineF32 {
 #pragma unroll
       for (int j = 0; j < MP; ++j) bigp |= (((pend >> j) & 1u) && lam[j] >= BIG_LAMBDA_F32) ? (1u << j) : 0u;
       pend &= ~bigp;
-      while (bigp != 0u) {
-        const int j = __builtin_ctz(bigp);
-        bigp &= bigp - 1u;
-        const Draw dr = ptrs_big_f64(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1,
-                                     (double)pick_m<MP>(lam, j));
-        cnt.w.ptrs += dr.iters;
-        cnt.w.uniforms += 2 * dr.iters;
-        if (dr.k < 0) return FLAG_PTRS;
+      if (bigp != 0u) {
 #pragma unroll
-        for (int r = 0; r < MP; ++r) K[r] = (r == j) ? (int32_t)dr.k : K[r];
+        for (int j = 0; j < MP; ++j) {
+          if ((bigp >> j) & 1u) {
+            const Draw dr = ptrs_big_f64(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1,
+                                         (double)lam[j]);
+            cnt.w.ptrs += dr.iters;
+            cnt.w.uniforms += 2 * dr.iters;
+            if (dr.k < 0) return FLAG_PTRS;
+            K[j] = (int32_t)dr.k;
+          }
+        }
       }
       // FP32 draws rejected at iteration 0: iteration `it` of reaction j uses
       // half j % 2 of block (grp + j / 2, it).  Each lane works through its
       // pending reactions in index order and moves to the next one as soon as
       // a draw is accepted, so the wavefront runs the largest per-lane total
       // of retries instead of one retry loop per pending reaction.
+      // The lane reads the mean of its current reaction from entry 0 of a
+      // register queue of its pending means (in index order), shifted when a
+      // draw is accepted: with a run-time index into lam the compiler would
+      // place lam in LDS and every read would wait for it.
+      constexpr int QN = MP <= 8 ? MP : 1;
+      float ql[QN];
+      if constexpr (MP <= 8) {
+        int nq = 0;
+#pragma unroll
+        for (int q = 0; q < QN; ++q) ql[q] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < MP; ++j) {
+          const bool on = (pend >> j) & 1u;
+#pragma unroll
+          for (int q = 0; q <= j; ++q) ql[q] = (on && nq == q) ? lam[j] : ql[q];
+          nq += on ? 1 : 0;
+        }
+      }
       {
         uint32_t it = 1;
         bool fail = false;
         while (pend != 0u) {
           const int j = __builtin_ctz(pend);
-          const PtrsF32 Pt = ptrs_setup_f32(pick_m<MP>(lam, j));
+          float lj;
+          if constexpr (MP <= 8) {
+            lj = ql[0];
+          } else {
+            lj = pick_m<MP>(lam, j);
+          }
+          const PtrsF32 Pt = ptrs_setup_f32(lj);
           const Words4 bw = key.block(grp + (uint64_t)(j >> 1), it);
           const float U = u24((j & 1) ? bw.w2 : bw.w0) - 0.5f;
           const float V = u24((j & 1) ? bw.w3 : bw.w1);
@@ -588,6 +614,8 @@ struct EngineF32 {
             for (int r = 0; r < MP; ++r) K[r] = (r == j) ? (int32_t)kf : K[r];
             pend &= pend - 1u;
             it = 1;
+#pragma unroll
+            for (int q = 0; q + 1 < QN; ++q) ql[q] = ql[q + 1];
           } else if (++it >= (uint32_t)PTRS_MAX_ITERS) {
             fail = true;
             pend = 0u;
