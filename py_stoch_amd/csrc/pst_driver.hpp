@@ -47,12 +47,14 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
+// x[idx] for a run-time idx in [0, NP), as an OR of masked elements: a select
+// chain is folded back into an indexed load by the compiler, which then moves
+// the per-lane array out of registers (into LDS or scratch).
 template <int NP>
 __device__ __forceinline__ int32_t pick_i(const int32_t (&x)[NP], int idx) {
-  int32_t r = x[0];
+  int32_t r = 0;
 #pragma unroll
-  for (int i = 1; i < NP; ++i)
-    if (idx == i) r = x[i];
+  for (int i = 0; i < NP; ++i) r |= x[i] & -(int32_t)(idx == i);
   return r;
 }
 

@@ -52,13 +52,13 @@ __device__ __forceinline__ float calc_g_f32(int h, int ht, int xi) {
   return 0.0f;
 }
 
-template <int NP, typename T>
-__device__ __forceinline__ T pick_m(const T (&x)[NP], int idx) {
-  T r = x[0];
+// x[idx] for a run-time idx in [0, NP) (floats), masked like pick_i
+template <int NP>
+__device__ __forceinline__ float pick_m(const float (&x)[NP], int idx) {
+  uint32_t r = 0;
 #pragma unroll
-  for (int i = 1; i < NP; ++i)
-    if (idx == i) r = x[i];
-  return r;
+  for (int i = 0; i < NP; ++i) r |= __builtin_bit_cast(uint32_t, x[i]) & (0u - (uint32_t)(idx == i));
+  return __builtin_bit_cast(float, r);
 }
 
 // log(1 + d) with the rounding of 1 + d corrected (Goldberg's method)

@@ -61,13 +61,14 @@ __device__ __forceinline__ double calc_g_f64(int h, int ht, int xi) {
   return 0.0;
 }
 
-template <int NP, typename T>
-__device__ __forceinline__ T pick(const T (&x)[NP], int idx) {
-  T r = x[0];
+// x[idx] for a run-time idx in [0, NP) (doubles), masked like pick_i (the bits
+// are copied, so the value is exact)
+template <int NP>
+__device__ __forceinline__ double pick(const double (&x)[NP], int idx) {
+  uint64_t r = 0;
 #pragma unroll
-  for (int i = 1; i < NP; ++i)
-    if (idx == i) r = x[i];
-  return r;
+  for (int i = 0; i < NP; ++i) r |= __builtin_bit_cast(uint64_t, x[i]) & (0ull - (uint64_t)(idx == i));
+  return __builtin_bit_cast(double, r);
 }
 
 // PTRS (Hoermann 1993) constants for one Poisson mean
