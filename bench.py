@@ -442,6 +442,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default launch shapes (skip the launch autotuner)")
+    ap.add_argument("--launch", default=None, metavar="SHIFT,SLICE",
+                    help="launch shape of the mixed plan (lanes_shift,ssa_slice of an autotune); implies --no-tune "
+                         "for it (to profile the shape a default run chose)")
     ap.add_argument("--acc-sets", type=int, default=16,
                     help="independent sets of 65536 simulations per path for the accuracy check")
     ap.add_argument("--workload", default="lotka-volterra",
@@ -493,6 +496,8 @@ def main():
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
+        if args.launch:
+            p_mixed.set_launch(*(int(v) for v in args.launch.split(",")))
         p_mixed.execute(O_m.data_ptr(), U, None, stream.cuda_stream)
         torch.cuda.synchronize()
         print("bench: traffic probe done (copy %d bytes, one mixed launch)" % COPY_BYTES, file=sys.stderr)
@@ -501,7 +506,11 @@ def main():
     # launch shapes (simulations per wavefront, SSA steps per pass) of every
     # plan, timed on the first half of the samples; results do not depend on them
     tune = None
-    if not args.no_tune:
+    if args.launch:
+        shift, slc = (int(v) for v in args.launch.split(","))
+        p_mixed.set_launch(shift, slc)
+        tune = {"mixed": {"lanes_shift": shift, "ssa_slice": slc, "set": True}}
+    elif not args.no_tune:
         print("bench: autotuning launch shapes", file=sys.stderr, flush=True)
         tune = {name: plan.autotune(O.data_ptr(), U, None, stream.cuda_stream, fraction=0.5)
                 for name, plan, O in (("mixed", p_mixed, O_m), ("fp64_plain", p_plain, O_f),

@@ -228,6 +228,9 @@ typedef struct pst_launch {
 } pst_launch;
 int pst_plan_autotune(pst_plan* plan, int32_t* const* O_dev, const int64_t* o_strides,
                       int32_t* const* flags_dev, void* stream, double fraction, pst_launch* chosen);
+/* Set a launch shape directly (a pst_launch of an earlier autotune):
+ * 0 <= lanes_shift <= 6, 0 <= ssa_slice <= 63.  No reference counterpart. */
+int pst_plan_set_launch(pst_plan* plan, int32_t lanes_shift, int32_t ssa_slice);
 
 /* largest network the compiled kernels accept (species, reactions) */
 void pst_max_network(int32_t* max_species, int32_t* max_reactions);

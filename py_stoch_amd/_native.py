@@ -133,6 +133,7 @@ EXPORTED_SYMBOLS = (
     "pst_last_error",
     "pst_abi_version",
     "pst_plan_autotune",
+    "pst_plan_set_launch",
 )
 
 _lib = None
@@ -171,6 +172,8 @@ def load():
                                     ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_void_p),
                                     ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(PstLaunch)]
     L.pst_plan_autotune.restype = ctypes.c_int
+    L.pst_plan_set_launch.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]
+    L.pst_plan_set_launch.restype = ctypes.c_int
     L.pst_plan_create.argtypes = [ctypes.POINTER(PstNetwork), ctypes.POINTER(PstConfig), _P_F64,
                                   _P_I32, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
     L.pst_plan_create.restype = ctypes.c_int
@@ -310,6 +313,10 @@ class Plan:
         """Re-key the random stream of the following executions."""
         _check(load().pst_plan_set_stream(self._h, int(seed), int(sim_offset)), "pst_plan_set_stream")
 
+    def set_launch(self, lanes_shift, ssa_slice):
+        """Use this launch shape (e.g. one an autotune returned) for later executions."""
+        _check(load().pst_plan_set_launch(self._h, int(lanes_shift), int(ssa_slice)), "pst_plan_set_launch")
+
     def autotune(self, O_dev_ptr, o_stride, flags_dev_ptr=None, stream_ptr=None, fraction=0.25):
         """Time the candidate launch shapes on the first `fraction` of the
         sample times (writing into O_dev_ptr) and keep the fastest; returns it."""
@@ -391,6 +398,10 @@ class BatchPlan:
         _check(load().pst_batch_plan_execute(self._h, Op, Sp, Fp,
                                              ctypes.c_void_p(int(stream_ptr)) if stream_ptr else None),
                "pst_batch_plan_execute")
+
+    def set_launch(self, lanes_shift, ssa_slice):
+        """Use this launch shape (e.g. one an autotune returned) for later executions."""
+        _check(load().pst_plan_set_launch(self._h, int(lanes_shift), int(ssa_slice)), "pst_plan_set_launch")
 
     def autotune(self, O_ptrs, strides, flag_ptrs=None, stream_ptr=None, fraction=0.25):
         """Time the candidate launch shapes on the first `fraction` of every

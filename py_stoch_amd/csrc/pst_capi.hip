@@ -729,6 +729,16 @@ int pst_plan_autotune(pst_plan* p, int32_t* const* O_dev, const int64_t* o_strid
   return PST_OK;
 }
 
+int pst_plan_set_launch(pst_plan* p, int32_t lanes_shift, int32_t ssa_slice) {
+  g_err[0] = 0;
+  if (!p) return fail(PST_EINVAL, "null plan");
+  if (lanes_shift < 0 || lanes_shift > 6) return fail(PST_EINVAL, "lanes_shift %d outside [0, 6]", lanes_shift);
+  if (ssa_slice < 0 || ssa_slice > 63) return fail(PST_EINVAL, "ssa_slice %d outside [0, 63]", ssa_slice);
+  p->lanes_shift = lanes_shift;
+  p->ssa_slice = ssa_slice;
+  return PST_OK;
+}
+
 int pst_plan_set_stream(pst_plan* p, uint64_t seed, uint64_t sim_offset) {
   g_err[0] = 0;
   if (!p) return fail(PST_EINVAL, "null plan");

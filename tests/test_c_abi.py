@@ -102,3 +102,8 @@ def test_autotune_argument_checks(native):
     out = native.PstLaunch()
     assert L.pst_plan_autotune(None, None, None, None, None, 0.5, ctypes.byref(out)) == -1  # PST_EINVAL
     assert b"null" in L.pst_last_error()
+
+
+def test_set_launch_argument_checks(native):
+    L = native.load()
+    assert L.pst_plan_set_launch(None, 0, 7) == -1  # PST_EINVAL: null plan
