@@ -214,3 +214,32 @@ def test_autotuned_plan_gives_the_same_results(native, oracle_mod, precision):
     if precision == 0:
         Oc, _, sc = oracle_mod.simulate(tabs, rates, a.x_0, method=0, t_max=4.0, I=a.I, E=a.E, n_sims=8192, seed=12)
         assert np.array_equal(g1, Oc) and s1["events"] == sc["events"]
+
+
+def test_autotuned_batch_plan_gives_the_same_results(native):
+    """A batch of two networks (one launch group each): after the autotuner the
+    trajectories and counters equal those of an untuned plan."""
+    from py_stoch_amd import workloads
+    specs = [workloads.lotka_volterra(U=4096)[0], workloads.schlogl(U=8192)[0]]
+    for s in specs:
+        s.kw["I"] = s.kw["I"][:40]
+        s.kw["t_max"] = float(s.kw["I"][-1])
+    print("case: autotune batch", flush=True)
+    bufs1 = [_DeviceArray((s.kappa, s.ita, s.n_sims)) for s in specs]
+    bufs2 = [_DeviceArray((s.kappa, s.ita, s.n_sims)) for s in specs]
+    try:
+        p1 = native.BatchPlan(workloads.jobs(specs, native.PRECISION_MIXED))
+        p2 = native.BatchPlan(workloads.jobs(specs, native.PRECISION_MIXED))
+        ptrs1, ptrs2 = [b.ptr.value for b in bufs1], [b.ptr.value for b in bufs2]
+        strides = [s.n_sims for s in specs]
+        ch = p1.autotune(ptrs1, strides, fraction=0.5)
+        assert ch["lanes_shift"] in (0, 1) and ch["ssa_slice"] in (3, 7, 15)
+        p1.execute(ptrs1, strides)
+        p2.execute(ptrs2, strides)
+        s1, s2 = p1.stats(), p2.stats()
+        for b1, b2 in zip(bufs1, bufs2):
+            assert np.array_equal(b1.get(), b2.get())
+    finally:
+        for b in bufs1 + bufs2:
+            b.free()
+    assert all(s1[k] == s2[k] for k in ("events", "leaps", "ssa_steps", "flagged", "flagged_events"))
