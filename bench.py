@@ -282,7 +282,7 @@ def _ks_rows(A, B):
 def other_workload(args):
     """Throughput of the mixed path and the FP64 path (both with the network
     compiled in) on one of the other configurations, with the accuracy
-    statistic that configuration is judged by."""
+    statistic that fits that configuration."""
     import torch
     from py_stoch_amd import _native, workloads
     torch.cuda.init()
