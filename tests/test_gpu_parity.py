@@ -45,6 +45,15 @@ def test_lotka_volterra_tau_leaping(native, oracle_mod):
     assert st["leaps"] > 0
 
 
+def test_lotka_volterra_full_size(native, oracle_mod):
+    """The throughput workload at its full size (65,536 simulations, t_end =
+    10, 1000 samples, seed 2) is bit-exact too, including the explosive
+    simulations stopped at a count of 2^30 and their flags."""
+    _, st = _compare(native, oracle_mod, models.lotka_volterra(), [0, 1], 10.0, 1000, 65536,
+                     native.METHOD_TAU_LEAPING, seed=2)
+    assert st["leaps"] > 0 and st["ssa_steps"] > 0 and st["flagged"] > 0
+
+
 def test_schlogl_tau_leaping(native, oracle_mod):
     _, st = _compare(native, oracle_mod, models.schlogl(), [0], 10.0, 100, 1024, native.METHOD_TAU_LEAPING, seed=3)
     assert st["leaps"] > 0 and st["ssa_steps"] > 0
