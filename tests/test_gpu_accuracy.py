@@ -42,6 +42,9 @@ def test_mixed_matches_fp64_on_the_full_workload(native):
     Om, fm, sm = native.simulate(tabs, rates, a.x_0, precision=native.PRECISION_MIXED, seed=2, **kw)
     Of, ff, sf = native.simulate(tabs, rates, a.x_0, precision=native.PRECISION_FP64, seed=102, **kw)
     assert sm["leaps"] > 0 and sm["ssa_steps"] > 0
+    x0 = np.asarray(a.x_0)[a.E][:, None]
+    assert (Om[:, 0, :] == x0).all() and (Of[:, 0, :] == x0).all()  # sample 0 is x0
+    assert (Om >= 0).all()
     # explosive simulations are rare on either path
     assert int((fm != 0).sum()) < 20 and int((ff != 0).sum()) < 20
     km, kf = _kept(Om, fm), _kept(Of, ff)
