@@ -442,9 +442,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default launch shapes (skip the launch autotuner)")
-    ap.add_argument("--launch", default=None, metavar="SHIFT,SLICE",
-                    help="launch shape of the mixed plan (lanes_shift,ssa_slice of an autotune); implies --no-tune "
-                         "for it (to profile the shape a default run chose)")
+    ap.add_argument("--launch", default=None, metavar="SHIFT,SLICE[,TRIM]",
+                    help="launch shape of the mixed plan (lanes_shift,ssa_slice,waves_trim of an autotune); implies "
+                         "--no-tune for it (to profile the shape a default run chose)")
     ap.add_argument("--acc-sets", type=int, default=16,
                     help="independent sets of 65536 simulations per path for the accuracy check")
     ap.add_argument("--workload", default="lotka-volterra",
@@ -503,13 +503,14 @@ def main():
         print("bench: traffic probe done (copy %d bytes, one mixed launch)" % COPY_BYTES, file=sys.stderr)
         return
 
-    # launch shapes (simulations per wavefront, SSA steps per pass) of every
-    # plan, timed on the first half of the samples; results do not depend on them
+    # launch shapes (simulations per wavefront, SSA steps per pass, resident
+    # wavefronts per SIMD) of every plan, timed on the first half of the samples; results do not depend on them
     tune = None
     if args.launch:
-        shift, slc = (int(v) for v in args.launch.split(","))
-        p_mixed.set_launch(shift, slc)
-        tune = {"mixed": {"lanes_shift": shift, "ssa_slice": slc, "set": True}}
+        shape = [int(v) for v in args.launch.split(",")] + [0]
+        shift, slc, trim = shape[:3]
+        p_mixed.set_launch(shift, slc, trim)
+        tune = {"mixed": {"lanes_shift": shift, "ssa_slice": slc, "waves_trim": trim, "set": True}}
     elif not args.no_tune:
         print("bench: autotuning launch shapes", file=sys.stderr, flush=True)
         tune = {name: plan.autotune(O.data_ptr(), U, None, stream.cuda_stream, fraction=0.5)

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define PST_ABI_VERSION 3
+#define PST_ABI_VERSION 4
 
 /* status codes */
 #define PST_OK 0
@@ -209,7 +209,8 @@ int pst_batch_plan_execute(pst_plan* plan, int32_t* const* O_dev, const int64_t*
 /*
  * Launch autotuner.  Times candidate launch shapes of a plan -- simulations
  * per wavefront (the library's choice, and half of it) x SSA-only steps per
- * pass while lanes sit in a burst (3, 7, 15) -- on the first `fraction` of
+ * pass while lanes sit in a burst (3, 7, 15) x resident wavefronts per SIMD
+ * (the kernel's occupancy limit, and one fewer) -- on the first `fraction` of
  * every job's sample times (0 < fraction <= 1; writes into the given
  * buffers, which need the shapes of pst_batch_plan_execute), and keeps the
  * fastest for the plan's later executions (another shape replaces the
@@ -223,14 +224,21 @@ int pst_batch_plan_execute(pst_plan* plan, int32_t* const* O_dev, const int64_t*
 typedef struct pst_launch {
   int32_t lanes_shift;        /* simulations per wavefront = library choice >> lanes_shift */
   int32_t ssa_slice;          /* SSA-only steps per pass (pure SSA: 2 ssa_slice + 1) */
+  int32_t waves_trim;         /* resident wavefronts per SIMD held back from the kernel's
+                                 occupancy limit (at least one stays): with fewer lanes
+                                 than simulations, the work queue refills lanes and a
+                                 wavefront no longer lasts as long as its slowest
+                                 simulation */
+  int32_t reserved;
   double kernel_ms;           /* kernel time of the chosen shape on the tuning horizon */
   double default_ms;          /* same for the library's default shape (0, 7) */
 } pst_launch;
 int pst_plan_autotune(pst_plan* plan, int32_t* const* O_dev, const int64_t* o_strides,
                       int32_t* const* flags_dev, void* stream, double fraction, pst_launch* chosen);
 /* Set a launch shape directly (a pst_launch of an earlier autotune):
- * 0 <= lanes_shift <= 6, 0 <= ssa_slice <= 63.  No reference counterpart. */
-int pst_plan_set_launch(pst_plan* plan, int32_t lanes_shift, int32_t ssa_slice);
+ * 0 <= lanes_shift <= 6, 0 <= ssa_slice <= 63, 0 <= waves_trim <= 7.  No
+ * reference counterpart. */
+int pst_plan_set_launch(pst_plan* plan, int32_t lanes_shift, int32_t ssa_slice, int32_t waves_trim);
 
 /* largest network the compiled kernels accept (species, reactions) */
 void pst_max_network(int32_t* max_species, int32_t* max_reactions);

@@ -133,7 +133,7 @@ int compile(const std::string& source, int precision, std::vector<char>* code, s
 }  // namespace
 
 std::string rtc_source(const HostNet& net, const JobParams& c, int precision,
-                       const std::vector<std::string>& hazards) {
+                       const std::vector<std::string>& hazards, int waves_per_eu) {
   const int N = net.N, M = net.M;
   std::vector<int8_t> v(N * M), ord(N * M), stp(N * M), E(N, 0);
   for (int i = 0; i < N * M; ++i) {
@@ -172,7 +172,9 @@ std::string rtc_source(const HostNet& net, const JobParams& c, int precision,
   s += "  }\n};\n}  // namespace pst\n";
   s += precision == PST_PRECISION_FP64 ? "#define PST_ENGINE EngineF64\n#define PST_KERNEL_NAME pst_net_fp64\n"
                                        : "#define PST_ENGINE EngineF32\n#define PST_KERNEL_NAME pst_net_mixed\n";
-  s += "extern \"C\" __global__ void __launch_bounds__(256)\n"
+  s += "extern \"C\" __global__ void __launch_bounds__(256)";
+  if (waves_per_eu > 0) s += " __attribute__((amdgpu_waves_per_eu(" + std::to_string(waves_per_eu) + ")))";
+  s += "\n"
        "PST_KERNEL_NAME(const pst::JobParams* __restrict__ jobs, int n_jobs, unsigned int* __restrict__ queue,\n"
        "                 unsigned long long* __restrict__ stats) {\n"
        "  pst::run_driver<pst::PST_ENGINE<pst::FixedRef<pst::NetGen>>, pst::FixedRef<pst::NetGen>>(jobs, n_jobs, "

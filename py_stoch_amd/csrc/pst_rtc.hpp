@@ -33,8 +33,11 @@ struct RtcKernel {
 // settings in `c` (N, M, kappa, n_c, eta, method, ssa_burst, n_rates are
 // read).  `hazards` empty: mass-action propensities from the table; else one
 // C expression per reaction (X(i), C(k), R(v), see pst_network.hazards).
+// waves_per_eu > 0 asks the compiler for register use that fits that many
+// wavefronts per SIMD (the kernel's arithmetic, and so its results, do not
+// change).
 std::string rtc_source(const HostNet& net, const JobParams& c, int precision,
-                       const std::vector<std::string>& hazards);
+                       const std::vector<std::string>& hazards, int waves_per_eu = 0);
 
 // Compile (or fetch from the cache) the kernel for `source` on the current
 // device.  Returns 0, or a PST_* status with a message in err.

@@ -38,6 +38,30 @@ def test_schlogl_tau_leaping(native, oracle_mod):
     _compare(native, oracle_mod, models.schlogl(), [0], 10.0, 100, 1024, native.METHOD_TAU_LEAPING, seed=3)
 
 
+def test_schlogl_launch_beyond_resident_wavefronts(native, oracle_mod):
+    """262,144 simulations need 4096 wavefronts, more than the plain Schloegl
+    network kernel holds resident, so the launch takes the build compiled for
+    three wavefronts per SIMD: still bit-exact."""
+    _, st = _compare(native, oracle_mod, models.schlogl(), [0], 10.0, 11, 262144, native.METHOD_TAU_LEAPING, seed=3)
+    assert st["leaps"] > 0 and st["ssa_steps"] > 0
+
+
+@pytest.mark.parametrize("method", [0, 2])
+def test_occupancy_build_equals_plain_build_mixed(native, method):
+    """Mixed precision: the last 1024 of 262,144 simulations (occupancy build)
+    equal the same simulations run on their own (sim_offset; 1024 simulations
+    fit the plain build) bit for bit."""
+    U, tail = 262144, 1024
+    a = models.make_args(models.schlogl(), E=[0], t_max=10.0, ita=11, U=U)
+    tabs, rates = tables_of(a), a.rate_fn(a.c)
+    kw = dict(method=method, precision=native.PRECISION_MIXED, specialize=True, t_max=10.0, I=a.I, E=a.E, seed=9)
+    print("case: mixed schlogl method=%d, %d sims vs the last %d alone" % (method, U, tail), flush=True)
+    O, f, _ = native.simulate(tabs, rates, a.x_0, n_sims=U, **kw)
+    O_t, f_t, _ = native.simulate(tabs, rates, a.x_0, n_sims=tail, sim_offset=U - tail, **kw)
+    assert np.array_equal(O[:, :, U - tail:], O_t)
+    assert np.array_equal(f[U - tail:], f_t)
+
+
 def test_birth_death_pure_and_forced_ssa(native, oracle_mod):
     _compare(native, oracle_mod, models.birth_death(), [0], 100.0, 101, 1024, native.METHOD_PURE_SSA, seed=0)
     _compare(native, oracle_mod, models.birth_death(), [0], 100.0, 101, 1024, native.METHOD_FORCED_SSA, seed=0)
@@ -200,7 +224,7 @@ def test_autotuned_plan_gives_the_same_results(native, oracle_mod, precision):
         p1 = native.Plan(tabs, rates, a.x_0, **kw)
         p2 = native.Plan(tabs, rates, a.x_0, **kw)
         ch = p1.autotune(O1.ptr.value, 8192, fraction=0.5)
-        assert ch["lanes_shift"] in (0, 1) and ch["ssa_slice"] in (3, 7, 15)
+        assert ch["lanes_shift"] in (0, 1) and ch["ssa_slice"] in (3, 7, 15) and ch["waves_trim"] in (0, 1)
         assert 0.0 < ch["kernel_ms"] <= ch["default_ms"]
         p1.execute(O1.ptr.value, 8192)
         p2.execute(O2.ptr.value, 8192)
@@ -214,6 +238,34 @@ def test_autotuned_plan_gives_the_same_results(native, oracle_mod, precision):
     if precision == 0:
         Oc, _, sc = oracle_mod.simulate(tabs, rates, a.x_0, method=0, t_max=4.0, I=a.I, E=a.E, n_sims=8192, seed=12)
         assert np.array_equal(g1, Oc) and s1["events"] == sc["events"]
+
+
+def test_waves_trim_gives_the_same_results(native):
+    """Holding one resident wavefront per SIMD back (262,144 pure-SSA
+    simulations: four resident wavefronts per SIMD become three, and the work
+    queue refills lanes) changes only the launch: trajectories, flags and
+    counters are identical."""
+    U = 262144
+    a = models.make_args(models.schlogl(), E=[0], t_max=10.0, ita=11, U=U)
+    kw = dict(method=2, t_max=10.0, I=a.I, E=a.E, n_sims=U, seed=4, precision=1, specialize=True)
+    tabs, rates = tables_of(a), a.rate_fn(a.c)
+    print("case: waves_trim 0 vs 1, mixed schlogl pure SSA, %d sims" % U, flush=True)
+    O1, O2 = _DeviceArray((1, 11, U)), _DeviceArray((1, 11, U))
+    F1, F2 = _DeviceArray((U,)), _DeviceArray((U,))
+    try:
+        p1 = native.Plan(tabs, rates, a.x_0, **kw)
+        p2 = native.Plan(tabs, rates, a.x_0, **kw)
+        p2.set_launch(0, 7, 1)
+        p1.execute(O1.ptr.value, U, F1.ptr.value)
+        p2.execute(O2.ptr.value, U, F2.ptr.value)
+        s1, s2 = p1.stats(), p2.stats()
+        g1, g2, f1, f2 = O1.get(), O2.get(), F1.get(), F2.get()
+    finally:
+        for b in (O1, O2, F1, F2):
+            b.free()
+    assert np.array_equal(g1, g2) and np.array_equal(f1, f2)
+    assert all(s1[k] == s2[k] for k in ("events", "ssa_steps", "flagged", "flagged_events"))
+    assert s1["ssa_steps"] > 0
 
 
 def test_autotuned_batch_plan_gives_the_same_results(native):
@@ -233,7 +285,7 @@ def test_autotuned_batch_plan_gives_the_same_results(native):
         ptrs1, ptrs2 = [b.ptr.value for b in bufs1], [b.ptr.value for b in bufs2]
         strides = [s.n_sims for s in specs]
         ch = p1.autotune(ptrs1, strides, fraction=0.5)
-        assert ch["lanes_shift"] in (0, 1) and ch["ssa_slice"] in (3, 7, 15)
+        assert ch["lanes_shift"] in (0, 1) and ch["ssa_slice"] in (3, 7, 15) and ch["waves_trim"] in (0, 1)
         p1.execute(ptrs1, strides)
         p2.execute(ptrs2, strides)
         s1, s2 = p1.stats(), p2.stats()
