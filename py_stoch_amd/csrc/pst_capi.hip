@@ -303,6 +303,16 @@ int kernel_kind(const pst_job& J, const pst::HostNet& h) {
 // wavefronts than the limit is the separate waves_trim of the launch shape.)
 constexpr int kOccupancyWaves = 3;
 
+// A plan with several kernel launches (size classes, network groups, the
+// large-network kernel) spreads them over the caller's stream and up to
+// kSideStreams more, forked from and joined back to the caller's stream with
+// events.  Every launch is a persistent grid sized for the whole chip, so on
+// one stream a launch waits for the slowest simulation of the one before it;
+// on separate streams its workgroups take the compute units the earlier
+// launch releases.  The kernels share no state but the atomically updated
+// counters, and no result depends on the order.
+constexpr int kSideStreams = 3;
+
 // jobs sharing one specialised kernel (same network and constant settings)
 struct FixedGroup {
   pst::RtcKernel k;
@@ -335,6 +345,10 @@ struct pst_plan {
   int lanes_shift = 0;        // launch shape (pst_plan_autotune)
   int ssa_slice = kSsaSlice;
   int waves_trim = 0;
+  // extra streams for plans with several kernel launches (created on first need)
+  hipStream_t side[kSideStreams] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kSideStreams] = {};
 };
 
 // resident 256-lane blocks per CU for a kernel whose occupancy limit is bpc,
@@ -368,6 +382,11 @@ void pst_plan_destroy(pst_plan* p) {
   if (p->d_stats) (void)hipFree(p->d_stats);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
+  for (int i = 0; i < kSideStreams; ++i) {
+    if (p->side[i]) (void)hipStreamDestroy(p->side[i]);
+    if (p->join[i]) (void)hipEventDestroy(p->join[i]);
+  }
+  if (p->fork) (void)hipEventDestroy(p->fork);
   delete p;
 }
 
@@ -630,9 +649,33 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
   hipLaunchKernelGGL(zero_u32, dim3((n_jobs + 255) / 256), dim3(256), 0, s, p->d_queue, n_jobs);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "zero kernels");
   if ((e = hipEventRecord(p->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+  // streams of the launches: launch i goes to the caller's stream (i % 4 ==
+  // 0) or to side stream i % 4 - 1, which first waits for the zeroed counters
+  int n_launch = (int)p->fixed.size() + (n_wave > 0 ? 1 : 0);
+  for (int c = 0; c < pst::NUM_CLASSES; ++c) n_launch += count[c] > 0 ? 1 : 0;
+  const int n_side = std::min(kSideStreams, n_launch - 1);
+  if (n_side > 0) {
+    if (!p->fork && (e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming)) != hipSuccess)
+      return hip_fail(e, "hipEventCreate");
+    for (int i = 0; i < n_side; ++i) {
+      if (!p->side[i] && (e = hipStreamCreateWithFlags(&p->side[i], hipStreamNonBlocking)) != hipSuccess)
+        return hip_fail(e, "hipStreamCreate");
+      if (!p->join[i] && (e = hipEventCreateWithFlags(&p->join[i], hipEventDisableTiming)) != hipSuccess)
+        return hip_fail(e, "hipEventCreate");
+    }
+    if ((e = hipEventRecord(p->fork, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    for (int i = 0; i < n_side; ++i)
+      if ((e = hipStreamWaitEvent(p->side[i], p->fork, 0)) != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
+  }
+  int li = 0;
+  auto next_stream = [&]() -> hipStream_t {
+    const int k = (li++) % (n_side + 1);
+    return k == 0 ? s : p->side[k - 1];
+  };
   int qoff = 0;
   for (int c = 0; c < pst::NUM_CLASSES; ++c) {
     if (count[c] == 0) continue;
+    const hipStream_t ls = next_stream();
     int64_t need = 0;
     for (const PlanJob& pj : p->jobs)
       if (pj.cls == c) need += ((int64_t)pj.p.n_sims + pj.p.lanes - 1) / pj.p.lanes;
@@ -641,8 +684,8 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
     const int waves = (int)(need < resident ? need : resident);
     // the queue counters of this class's jobs are queue[qoff .. qoff + count)
     const int rc = (p->precision == PST_PRECISION_MIXED)
-                       ? pst::launch_mixed(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, s)
-                       : pst::launch_fp64(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, s);
+                       ? pst::launch_mixed(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, ls)
+                       : pst::launch_fp64(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, ls);
     if (rc < 0) return fail(PST_ETOOLARGE, "no compiled kernel for size class %d", c);
     if (rc != 0) return hip_fail((hipError_t)rc, "simulation kernel launch");
     qoff += count[c];
@@ -654,7 +697,8 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
     const int64_t resident = (int64_t)trimmed(k->blocks_per_cu, p->waves_trim) * p->cu_count * 4;
     const int waves = (int)(need < resident ? need : resident);
     const int cnt = (int)g.jobs.size();
-    if ((e = pst::rtc_launch(*k, (const pst::JobParams*)g.d_jobs, cnt, p->d_queue + qoff, p->d_stats, waves, s)) !=
+    const hipStream_t ls = next_stream();
+    if ((e = pst::rtc_launch(*k, (const pst::JobParams*)g.d_jobs, cnt, p->d_queue + qoff, p->d_stats, waves, ls)) !=
         hipSuccess)
       return hip_fail(e, "specialised kernel launch");
     qoff += cnt;
@@ -663,9 +707,13 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
     // one simulation per wavefront at a time
     const int64_t resident = (int64_t)trimmed(pst::blocks_per_cu_wave(), p->waves_trim) * p->cu_count * 4;
     const int waves = (int)(wave_sims < resident ? wave_sims : resident);
-    const int rc = pst::launch_wave(p->d_wave, n_wave, p->d_queue + qoff, p->d_stats, waves, s);
+    const int rc = pst::launch_wave(p->d_wave, n_wave, p->d_queue + qoff, p->d_stats, waves, next_stream());
     if (rc != 0) return hip_fail((hipError_t)rc, "large-network kernel launch");
     qoff += n_wave;
+  }
+  for (int i = 0; i < n_side; ++i) {
+    if ((e = hipEventRecord(p->join[i], p->side[i])) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    if ((e = hipStreamWaitEvent(s, p->join[i], 0)) != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
   }
   if ((e = hipEventRecord(p->ev1, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
   p->executed = true;
