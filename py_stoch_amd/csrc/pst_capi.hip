@@ -355,6 +355,16 @@ struct pst_plan {
 // after the plan's waves_trim (one block = one wavefront per SIMD)
 static int trimmed(int bpc, int trim) { return std::max(1, bpc - trim); }
 
+// Wait for every launch of the plan that may still read its descriptors: the
+// caller's stream, and the side streams (an execute that failed between fork
+// and join leaves work on them that the caller's stream does not wait for).
+static hipError_t sync_plan(pst_plan* p, hipStream_t s) {
+  hipError_t e = hipStreamSynchronize(s);
+  for (int i = 0; i < kSideStreams && e == hipSuccess; ++i)
+    if (p->side[i]) e = hipStreamSynchronize(p->side[i]);
+  return e;
+}
+
 extern "C" {
 
 const char* pst_last_error(void) { return g_err; }
@@ -592,7 +602,7 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
       p->d_jobs_bytes[c] = bytes.size();
     }
     // the descriptor array may still be read by an earlier launch of this plan
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    if ((e = sync_plan(p, s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     if ((e = hipMemcpy(p->d_jobs[c], bytes.data(), bytes.size(), hipMemcpyHostToDevice)) != hipSuccess)
       return hip_fail(e, "hipMemcpy(jobs)");
     p->uploaded[c] = bytes;
@@ -611,7 +621,7 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
       if ((e = hipMalloc(&g.d_jobs, bytes.size())) != hipSuccess) return hip_fail(e, "hipMalloc(jobs)");
       g.d_bytes = bytes.size();
     }
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    if ((e = sync_plan(p, s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     if ((e = hipMemcpy(g.d_jobs, bytes.data(), bytes.size(), hipMemcpyHostToDevice)) != hipSuccess)
       return hip_fail(e, "hipMemcpy(jobs)");
     g.uploaded = bytes;
@@ -639,7 +649,7 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
         if ((e = hipMalloc(&p->d_wave, bytes.size())) != hipSuccess) return hip_fail(e, "hipMalloc(jobs)");
         p->d_wave_bytes = bytes.size();
       }
-      if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+      if ((e = sync_plan(p, s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
       if ((e = hipMemcpy(p->d_wave, bytes.data(), bytes.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "hipMemcpy(jobs)");
       p->uploaded_wave = bytes;
