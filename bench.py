@@ -305,7 +305,11 @@ def other_workload(args):
     p_mix = _native.BatchPlan(workloads.jobs(specs, _native.PRECISION_MIXED))
     p_f64 = _native.BatchPlan(workloads.jobs(specs, _native.PRECISION_FP64))
     tune = None
-    if not args.no_tune:
+    if args.launch:
+        shift, slc, trim = ([int(v) for v in args.launch.split(",")] + [0])[:3]
+        p_mix.set_launch(shift, slc, trim)
+        tune = {"mixed": {"lanes_shift": shift, "ssa_slice": slc, "waves_trim": trim, "set": True}}
+    elif not args.no_tune:
         print("bench: %s, autotuning launch shapes" % wl, file=sys.stderr, flush=True)
         ptrs, strides = [O.data_ptr() for O in O_list], [s.n_sims for s in specs]
         tune = {"mixed": p_mix.autotune(ptrs, strides, None, stream.cuda_stream, fraction=TUNE_FRACTION),
