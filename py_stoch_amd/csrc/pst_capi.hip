@@ -46,10 +46,12 @@ constexpr int kMaxReactions = pst::WAVE_MAX_M;
 constexpr int kSsaSlice = 7;
 constexpr int kMaxParams = 256;
 
-// A compiled hazard may use X(i), C(k), R(v), numbers, + - * / ( ) , and these
-// functions; anything else (statements, preprocessor lines, other names) is
-// rejected before the expression reaches the compiler.
-bool hazard_ok(const char* h, std::string* why) {
+// A compiled hazard may use X(i) (0 <= i < N), C(k) (0 <= k < n_params),
+// R(v), numbers, + - * / ( ) , and these functions; anything else
+// (statements, preprocessor lines, comments, other names, an index that is
+// not an integer literal in range) is rejected before the expression reaches
+// the compiler.
+bool hazard_ok(const char* h, int n_species, int n_params, std::string* why) {
   static const char* const kNames[] = {"X", "C", "R", "exp", "log", "sqrt", "pow", "fabs", "fmin", "fmax",
                                        "tanh", "exp2", "log2", "log10", "cbrt", "floor", "ceil"};
   int depth = 0;
@@ -66,6 +68,29 @@ bool hazard_ok(const char* h, std::string* why) {
         return false;
       }
       p = q;
+      if (name == "X" || name == "C") {
+        // the argument is a decimal integer literal inside the table's range
+        const int limit = name == "X" ? n_species : n_params;
+        const char* r = p;
+        if (*r != '(') {
+          *why = name + " takes an index in parentheses";
+          return false;
+        }
+        ++r;
+        long idx = 0;
+        int digits = 0;
+        while (std::isdigit((unsigned char)*r) && digits < 9) idx = idx * 10 + (*r++ - '0'), ++digits;
+        if (digits == 0 || *r != ')') {
+          *why = name + "(...) needs a non-negative integer literal";
+          return false;
+        }
+        if (idx >= limit) {
+          *why = name + "(" + std::to_string(idx) + ") is out of range (" + (name == "X" ? "N" : "n_params") +
+                 " = " + std::to_string(limit) + ")";
+          return false;
+        }
+        p = r + 1;
+      }
     } else if (std::isdigit((unsigned char)ch) || ch == '.') {
       // number, with an optional exponent
       while (std::isdigit((unsigned char)*p) || *p == '.') ++p;
@@ -81,6 +106,9 @@ bool hazard_ok(const char* h, std::string* why) {
         return false;
       }
       ++p;
+    } else if (ch == '/' && (p[1] == '/' || p[1] == '*')) {
+      *why = "comments are not allowed";
+      return false;
     } else if (std::strchr("+-*/, ", ch)) {
       ++p;
     } else {
@@ -146,7 +174,7 @@ int prepare(const pst_network* net, const pst_config* cfg, const double* rates, 
     for (int j = 0; j < M; ++j) {
       std::string why;
       if (!net->hazards[j] || !net->hazards[j][0]) return fail(PST_EINVAL, "hazard %d is empty", j);
-      if (!hazard_ok(net->hazards[j], &why)) return fail(PST_EINVAL, "hazard %d: %s", j, why.c_str());
+      if (!hazard_ok(net->hazards[j], N, net->n_params, &why)) return fail(PST_EINVAL, "hazard %d: %s", j, why.c_str());
     }
   }
   const int n_rates = custom ? net->n_params : M;
@@ -341,7 +369,8 @@ struct pst_plan {
   unsigned int* d_queue = nullptr;
   unsigned long long* d_stats = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool executed = false;
+  bool executed = false;      // the last execute enqueued every launch
+  bool enqueued = false;      // ev1 marks the end of all work of the last execute
   int lanes_shift = 0;        // launch shape (pst_plan_autotune)
   int ssa_slice = kSsaSlice;
   int waves_trim = 0;
@@ -356,13 +385,63 @@ struct pst_plan {
 static int trimmed(int bpc, int trim) { return std::max(1, bpc - trim); }
 
 // Wait for every launch of the plan that may still read its descriptors: the
-// caller's stream, and the side streams (an execute that failed between fork
-// and join leaves work on them that the caller's stream does not wait for).
+// caller's stream, the end of the previous execute (recorded on the stream
+// that execute was given, which may be another one, after its side streams
+// were joined back into it, also when it failed part way), and the side
+// streams themselves.
 static hipError_t sync_plan(pst_plan* p, hipStream_t s) {
   hipError_t e = hipStreamSynchronize(s);
+  if (e == hipSuccess && p->enqueued) e = hipEventSynchronize(p->ev1);
   for (int i = 0; i < kSideStreams && e == hipSuccess; ++i)
     if (p->side[i]) e = hipStreamSynchronize(p->side[i]);
   return e;
+}
+
+// The kernel launches of one execute: size classes, network groups, then the
+// large-network kernel, each on the stream next_stream() gives.
+template <class NextStream>
+static int enqueue_launches(pst_plan* p, const int* count, int n_wave, int64_t wave_sims, NextStream& next_stream) {
+  hipError_t e;
+  int qoff = 0;
+  for (int c = 0; c < pst::NUM_CLASSES; ++c) {
+    if (count[c] == 0) continue;
+    const hipStream_t ls = next_stream();
+    int64_t need = 0;
+    for (const PlanJob& pj : p->jobs)
+      if (pj.cls == c) need += ((int64_t)pj.p.n_sims + pj.p.lanes - 1) / pj.p.lanes;
+    const int bpc = (p->precision == PST_PRECISION_MIXED) ? pst::blocks_per_cu_mixed(c) : pst::blocks_per_cu_fp64(c);
+    const int64_t resident = (int64_t)trimmed(bpc, p->waves_trim) * p->cu_count * 4;
+    const int waves = (int)(need < resident ? need : resident);
+    // the queue counters of this class's jobs are queue[qoff .. qoff + count)
+    const int rc = (p->precision == PST_PRECISION_MIXED)
+                       ? pst::launch_mixed(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, ls)
+                       : pst::launch_fp64(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, ls);
+    if (rc < 0) return fail(PST_ETOOLARGE, "no compiled kernel for size class %d", c);
+    if (rc != 0) return hip_fail((hipError_t)rc, "simulation kernel launch");
+    qoff += count[c];
+  }
+  for (FixedGroup& g : p->fixed) {
+    int64_t need = 0;
+    for (int q : g.jobs) need += ((int64_t)p->jobs[q].p.n_sims + p->jobs[q].p.lanes - 1) / p->jobs[q].p.lanes;
+    const pst::RtcKernel* k = g.use_occ ? &g.k_occ : &g.k;
+    const int64_t resident = (int64_t)trimmed(k->blocks_per_cu, p->waves_trim) * p->cu_count * 4;
+    const int waves = (int)(need < resident ? need : resident);
+    const int cnt = (int)g.jobs.size();
+    const hipStream_t ls = next_stream();
+    if ((e = pst::rtc_launch(*k, (const pst::JobParams*)g.d_jobs, cnt, p->d_queue + qoff, p->d_stats, waves, ls)) !=
+        hipSuccess)
+      return hip_fail(e, "specialised kernel launch");
+    qoff += cnt;
+  }
+  if (n_wave > 0) {
+    // one simulation per wavefront at a time
+    const int64_t resident = (int64_t)trimmed(pst::blocks_per_cu_wave(), p->waves_trim) * p->cu_count * 4;
+    const int waves = (int)(wave_sims < resident ? wave_sims : resident);
+    const int rc = pst::launch_wave(p->d_wave, n_wave, p->d_queue + qoff, p->d_stats, waves, next_stream());
+    if (rc != 0) return hip_fail((hipError_t)rc, "large-network kernel launch");
+    qoff += n_wave;
+  }
+  return PST_OK;
 }
 
 extern "C" {
@@ -655,6 +734,11 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
       p->uploaded_wave = bytes;
     }
   }
+  // The queue and counters are shared by every execute of the plan: the
+  // previous one, on whichever stream it was given, must be done with them
+  // before they are cleared.
+  p->executed = false;
+  if (p->enqueued && (e = hipStreamWaitEvent(s, p->ev1, 0)) != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
   hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, s, p->d_stats, (int)pst::ST_COUNT);
   hipLaunchKernelGGL(zero_u32, dim3((n_jobs + 255) / 256), dim3(256), 0, s, p->d_queue, n_jobs);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "zero kernels");
@@ -682,50 +766,19 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
     const int k = (li++) % (n_side + 1);
     return k == 0 ? s : p->side[k - 1];
   };
-  int qoff = 0;
-  for (int c = 0; c < pst::NUM_CLASSES; ++c) {
-    if (count[c] == 0) continue;
-    const hipStream_t ls = next_stream();
-    int64_t need = 0;
-    for (const PlanJob& pj : p->jobs)
-      if (pj.cls == c) need += ((int64_t)pj.p.n_sims + pj.p.lanes - 1) / pj.p.lanes;
-    const int bpc = (p->precision == PST_PRECISION_MIXED) ? pst::blocks_per_cu_mixed(c) : pst::blocks_per_cu_fp64(c);
-    const int64_t resident = (int64_t)trimmed(bpc, p->waves_trim) * p->cu_count * 4;
-    const int waves = (int)(need < resident ? need : resident);
-    // the queue counters of this class's jobs are queue[qoff .. qoff + count)
-    const int rc = (p->precision == PST_PRECISION_MIXED)
-                       ? pst::launch_mixed(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, ls)
-                       : pst::launch_fp64(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, ls);
-    if (rc < 0) return fail(PST_ETOOLARGE, "no compiled kernel for size class %d", c);
-    if (rc != 0) return hip_fail((hipError_t)rc, "simulation kernel launch");
-    qoff += count[c];
-  }
-  for (FixedGroup& g : p->fixed) {
-    int64_t need = 0;
-    for (int q : g.jobs) need += ((int64_t)p->jobs[q].p.n_sims + p->jobs[q].p.lanes - 1) / p->jobs[q].p.lanes;
-    const pst::RtcKernel* k = g.use_occ ? &g.k_occ : &g.k;
-    const int64_t resident = (int64_t)trimmed(k->blocks_per_cu, p->waves_trim) * p->cu_count * 4;
-    const int waves = (int)(need < resident ? need : resident);
-    const int cnt = (int)g.jobs.size();
-    const hipStream_t ls = next_stream();
-    if ((e = pst::rtc_launch(*k, (const pst::JobParams*)g.d_jobs, cnt, p->d_queue + qoff, p->d_stats, waves, ls)) !=
-        hipSuccess)
-      return hip_fail(e, "specialised kernel launch");
-    qoff += cnt;
-  }
-  if (n_wave > 0) {
-    // one simulation per wavefront at a time
-    const int64_t resident = (int64_t)trimmed(pst::blocks_per_cu_wave(), p->waves_trim) * p->cu_count * 4;
-    const int waves = (int)(wave_sims < resident ? wave_sims : resident);
-    const int rc = pst::launch_wave(p->d_wave, n_wave, p->d_queue + qoff, p->d_stats, waves, next_stream());
-    if (rc != 0) return hip_fail((hipError_t)rc, "large-network kernel launch");
-    qoff += n_wave;
-  }
+  const int rc_launch = enqueue_launches(p, count, n_wave, wave_sims, next_stream);
+  // join the side streams back into the caller's stream and mark the end of
+  // this execute, after a failed launch too: the next execute and
+  // pst_plan_read_stats wait for ev1
+  hipError_t ej = hipSuccess;
   for (int i = 0; i < n_side; ++i) {
-    if ((e = hipEventRecord(p->join[i], p->side[i])) != hipSuccess) return hip_fail(e, "hipEventRecord");
-    if ((e = hipStreamWaitEvent(s, p->join[i], 0)) != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
+    if ((e = hipEventRecord(p->join[i], p->side[i])) != hipSuccess && ej == hipSuccess) ej = e;
+    if ((e = hipStreamWaitEvent(s, p->join[i], 0)) != hipSuccess && ej == hipSuccess) ej = e;
   }
-  if ((e = hipEventRecord(p->ev1, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+  if ((e = hipEventRecord(p->ev1, s)) != hipSuccess && ej == hipSuccess) ej = e;
+  if (e == hipSuccess) p->enqueued = true;
+  if (rc_launch != PST_OK) return rc_launch;
+  if (ej != hipSuccess) return hip_fail(ej, "hipEventRecord / hipStreamWaitEvent");
   p->executed = true;
   return PST_OK;
 }

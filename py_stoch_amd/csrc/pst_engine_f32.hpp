@@ -540,7 +540,11 @@ struct EngineF32 {
             const float p2 = p1 * lam[j] * r2;
             const float F2 = F1 + p2;
             K[j] = live[j] ? ((u[j] > F1) ? it + 2 : it + 1) : K[j];
-            live[j] = live[j] && (u[j] > F2);
+            // a search also ends where the FP32 sum stops growing: past the
+            // mode every later term is smaller still, and a u above that sum
+            // (u24 can be 1.0, the sum can end a few ulps below it) would
+            // otherwise run to INV_MAX_K
+            live[j] = live[j] && (u[j] > F2) && (F2 > Fc[j]);
             any_live = any_live | live[j];
             p[j] = p2;
             Fc[j] = F2;

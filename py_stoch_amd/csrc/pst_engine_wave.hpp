@@ -392,7 +392,9 @@ struct EngineWave {
             while (u > Fc && n < INV_MAX_K) {
               n += 1;
               p = p * l * fast_rcp((float)n);
-              Fc += p;
+              const float Fn = Fc + p;
+              if (!(Fn > Fc)) break;  // the FP32 sum stopped growing (far tail; u may be 1.0)
+              Fc = Fn;
             }
             cnt.w.inv += (unsigned int)n;
             kk = n;

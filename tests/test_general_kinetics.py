@@ -60,3 +60,11 @@ def test_c_abi_validates_hazards(native):
     assert "unbalanced" in _plan_error(native, ["(C(0)", "C(1)"])
     assert "not allowed" in _plan_error(native, ["C(0)", "#include"])
     assert "empty" in _plan_error(native, ["", "C(1)"])
+    # indexes are integer literals inside the network's species / parameter counts
+    assert "X(5) is out of range" in _plan_error(native, ["C(0)*X(5)", "C(1)"])
+    assert "C(7) is out of range" in _plan_error(native, ["C(7)", "C(1)"])
+    assert "integer literal" in _plan_error(native, ["C(0)*X(0+1)", "C(1)"])
+    assert "integer literal" in _plan_error(native, ["C(0)*X(-1)", "C(1)"])
+    # a comment could swallow the next hazard's assignment
+    assert "comments" in _plan_error(native, ["C(0) /* ", "*/ C(1)"])
+    assert "comments" in _plan_error(native, ["C(0) // x", "C(1)"])
