@@ -112,6 +112,13 @@ def lib():
                                    ctypes.c_int]
         L.orc_simulate.restype = ctypes.c_int
         L.orc_max_threads.restype = ctypes.c_int
+        L.orc_set_plant.argtypes = [ctypes.c_int]
+        L.orc_set_plant.restype = None
+        L.orc_critical_draws.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.c_int, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64,
+                                         ctypes.c_int64, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_int32)]
+        L.orc_critical_draws.restype = None
         _lib = L
     return _lib
 
@@ -154,6 +161,39 @@ def lgamma_int(x):
 
 def max_threads():
     return lib().orc_max_threads()
+
+
+def critical_draws(a, critical, tau1, seed, n, sim=0):
+    """n draws of the critical part of a leap (tau2 and SingleCriticalReaction,
+    P1_P2.py:175-220) at propensities a with critical flags `critical`:
+    returns (tau[n], picked reaction[n], -1 when tau1 <= tau2)."""
+    a = np.ascontiguousarray(a, np.float64)
+    crit = np.ascontiguousarray(critical, np.int32)
+    tau = np.empty(n, np.float64)
+    j = np.empty(n, np.int32)
+    lib().orc_critical_draws(_p(a, ctypes.c_double), _p(crit, ctypes.c_int32), a.size, float(tau1), seed, sim, n,
+                             _p(tau, ctypes.c_double), _p(j, ctypes.c_int32))
+    return tau, j
+
+
+class planted:
+    """Context manager that plants a known mistake in the oracle's critical
+    branch for the duration of a block (tests only):
+    ``"tau2_a0"`` draws tau2 with a0 instead of a0c, ``"pick_a0"`` normalises
+    the critical pick by a0 instead of a0c."""
+
+    _CODES = {"tau2_a0": 1, "pick_a0": 2}
+
+    def __init__(self, what):
+        self.code = self._CODES[what]
+
+    def __enter__(self):
+        lib().orc_set_plant(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_plant(0)
+        return False
 
 
 def simulate(tables, rates, x0, method=METHOD_TAU_LEAPING, n_c=10, eta=0.03, ssa_burst=100,

@@ -367,6 +367,68 @@ static void fill_rest(const orc_config* c, int32_t* O, int64_t U, int64_t sim, i
  * One simulation through the reference's per-thread control flow.
  * Returns the flag bits.
  */
+/* Planted mistakes, for the tests that show the exact-law checks catch them
+ * (set only from tests; 0 everywhere else):
+ *   1: tau2 drawn with the total propensity a0 instead of a0c (P1_P2.py:200-204)
+ *   2: the critical pick normalised by a0 instead of a0c (P1_P2.py:500)      */
+static int orc_plant = 0;
+void orc_set_plant(int p) { orc_plant = p; }
+
+/* The critical part of a leap (P1_P2.py:175-220): a0c = sum of the critical
+ * propensities; if a0c > 0, tau2 = (1/a0c) ln(1/rho1) from the next block;
+ * tau = fminf(tau1, tau2); if tau1 > tau2, SingleCriticalReaction
+ * (P1_P2.py:488-507) picks the first critical j with sum a_r / a0c >= rho
+ * (rho from the same block) and stores it in *jc (else -1).  The reference
+ * falls off the end if round-off leaves the sum below rho: the last critical
+ * reaction with a_j > 0 is taken instead.  Returns tau. */
+static double critical_draw(orc_rng* g, const double* a, const int32_t* Xeta, int M, double a0, double tau1,
+                            int* jc)
+{
+    double a0c = 0.0;
+    for (int j = 0; j < M; ++j) if (Xeta[j] == 1) a0c = a0c + a[j];
+    double tau2 = INFINITY;
+    uint32_t wc[4] = {0, 0, 0, 0};
+    if (a0c > 0.0) {
+        rng_block(g, g->b, 0, wc);
+        g->b += 1;
+        double rho1 = u53(wc[0], wc[1]);
+        tau2 = (1.0 / (orc_plant == 1 ? a0 : a0c)) * orc_log(1.0 / rho1);
+    }
+    double tau = (double)fminf((float)tau1, (float)tau2);
+    *jc = -1;
+    if (tau1 > tau2) {
+        double rho = u53(wc[2], wc[3]);
+        double rc = 0.0;
+        int last = -1;
+        for (int r = 0; r < M; ++r) {
+            if (Xeta[r] == 1) {
+                if (a[r] > 0.0) last = r;
+                rc = rc + a[r] / (orc_plant == 2 ? a0 : a0c);
+                if (rc >= rho) { *jc = r; break; }
+            }
+        }
+        if (*jc < 0) *jc = last;
+    }
+    return tau;
+}
+
+/* n independent critical draws (test hook): draw i uses simulation id
+ * `sim` + i of stream `seed` from block 0, as the first leap of that
+ * simulation would.  tau_out[i] = tau, j_out[i] = picked reaction or -1. */
+void orc_critical_draws(const double* a, const int32_t* Xeta, int M, double tau1, uint64_t seed, uint64_t sim,
+                        int64_t n, double* tau_out, int32_t* j_out)
+{
+    double a0 = 0.0;
+    for (int j = 0; j < M; ++j) a0 = a0 + a[j];
+    for (int64_t i = 0; i < n; ++i) {
+        orc_rng g;
+        rng_init(&g, seed, sim + (uint64_t)i);
+        int jc = -1;
+        tau_out[i] = critical_draw(&g, a, Xeta, M, a0, tau1, &jc);
+        j_out[i] = jc;
+    }
+}
+
 static int simulate_one(const orc_network* net, const double* k, const int32_t* x0,
                         const orc_config* c, int64_t U, int64_t sim, int32_t* O, orc_stats* st)
 {
@@ -491,34 +553,9 @@ static int simulate_one(const orc_network* net, const double* k, const int32_t* 
             } else {
                 Q = 1;
                 for (int j = 0; j < M; ++j) K[j] = 0;
-                double a0c = 0.0;
-                for (int j = 0; j < M; ++j) if (Xeta[j] == 1) a0c = a0c + a[j];
-                double tau2 = INFINITY;
-                uint32_t wc[4] = {0, 0, 0, 0};
-                if (a0c > 0.0) {
-                    rng_block(&g, g.b, 0, wc);
-                    g.b += 1;
-                    double rho1 = u53(wc[0], wc[1]);
-                    tau2 = (1.0 / a0c) * orc_log(1.0 / rho1);
-                }
-                double tau = (double)fminf((float)tau1, (float)tau2);
-                if (tau1 > tau2) {
-                    /* SingleCriticalReaction (P1_P2.py:488-507); the reference
-                     * falls off the end if round-off leaves rcount < rho: the
-                     * last critical reaction with a_j > 0 is taken instead. */
-                    double rho = u53(wc[2], wc[3]);
-                    double rc = 0.0;
-                    int jc = -1, last = -1;
-                    for (int r = 0; r < M; ++r) {
-                        if (Xeta[r] == 1) {
-                            if (a[r] > 0.0) last = r;
-                            rc = rc + a[r] / a0c;
-                            if (rc >= rho) { jc = r; break; }
-                        }
-                    }
-                    if (jc < 0) jc = last;
-                    K[jc] = 1;
-                }
+                int jc = -1;
+                double tau = critical_draw(&g, a, Xeta, M, a0, tau1, &jc);
+                if (jc >= 0) K[jc] = 1;
                 if (!(tau < INFINITY)) {
                     flags |= ORC_FLAG_TAU_INF;
                     fill_rest(c, O, U, sim, F, x, 1);
