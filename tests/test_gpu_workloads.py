@@ -1,0 +1,127 @@
+"""The parameter-sweep (C4) and random-network (C5) workloads on the GPU.
+
+C4: workloads.sweep() in full -- 4096 parameter sets (Lotka-Volterra and
+Schlogl families, every rate constant scaled by 10^U(-1,1), which includes
+the explosive Lotka-Volterra sets), 256 simulations each, one batched plan.
+The FP64 batch is bit-exact against the oracle on the first 8 simulations of
+every set (same global simulation ids, so the same random streams), and
+mixed precision agrees with FP64 set by set in distribution.
+
+C5: eight networks of workloads.random_networks() -- the largest one (49
+species, 193 reactions) and networks 0-6 -- under the same step budget
+(20,000 leap decisions + SSA steps per simulation), mixed precision against
+the FP64 oracle in distribution.  A simulation that runs out of steps keeps
+its samples up to that point (the rest are zero), so each sample time is
+compared over the simulations, on both sides, that were still running there
+(every test marked gpu)."""
+import os
+
+import numpy as np
+import pytest
+
+from py_stoch_amd import workloads
+
+import laws
+
+pytestmark = pytest.mark.gpu
+
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def _ks_crit(n1, n2, alpha=1e-6):
+    return np.sqrt(-np.log(alpha / 2) / 2) * np.sqrt((n1 + n2) / (n1 * n2))
+
+
+def test_sweep_fp64_bit_exact_and_mixed_per_set(native, oracle_mod):
+    specs = workloads.sweep()
+    print("case: sweep fp64 batch (%d jobs, %d simulations)" % (len(specs), sum(s.n_sims for s in specs)),
+          flush=True)
+    O64, F64, st64 = native.batch_simulate(workloads.jobs(specs, native.PRECISION_FP64))
+    print("case: sweep mixed batch", flush=True)
+    Omx, Fmx, stmx = native.batch_simulate(workloads.jobs(specs, native.PRECISION_MIXED))
+    assert st64["flagged"] > 0  # the explosive Lotka-Volterra sets stop at 2^30
+    prefix = 8
+    ks_bad, z_all, compared = 0, [], 0
+    for s, Og, Fg, Om, Fm in zip(specs, O64, F64, Omx, Fmx):
+        per = s.kw["sims_per_set"]
+        n_sets = s.n_sims // per
+        print("case: sweep %s oracle on %d x %d simulations" % (s.name, n_sets, prefix), flush=True)
+        for p in range(n_sets):
+            kw = dict(s.kw)
+            kw.update(n_sims=prefix, sims_per_set=prefix, sim_offset=p * per)
+            Oc, Fc, _ = oracle_mod.simulate(s.tables, s.rates[p:p + 1], s.x0[p:p + 1], threads=THREADS, **kw)
+            sl = slice(p * per, p * per + prefix)
+            if not np.array_equal(Og[:, :, sl], Oc) or not np.array_equal(Fg[sl], Fc):
+                bad = np.argwhere(Og[:, :, sl] != Oc)
+                raise AssertionError("%s set %d: FP64 GPU differs from the oracle at %s" % (s.name, p, bad[:3]))
+        # mixed vs FP64, per set and recorded species, at the middle and last sample
+        for p in range(n_sets):
+            sl = slice(p * per, (p + 1) * per)
+            a_ok, b_ok = Fm[sl] == 0, Fg[sl] == 0
+            if a_ok.sum() < 64 or b_ok.sum() < 64:
+                continue  # explosive sets: most simulations flagged on both sides
+            for k in range(s.kappa):
+                for f in (s.ita // 2, s.ita - 1):
+                    x, y = Om[k, f, sl][a_ok].astype(np.float64), Og[k, f, sl][b_ok].astype(np.float64)
+                    compared += 1
+                    if laws.ks2(x, y) > _ks_crit(x.size, y.size):
+                        ks_bad += 1
+                    se = np.sqrt(x.var() / x.size + y.var() / y.size)
+                    if se > 0:
+                        z_all.append(abs(x.mean() - y.mean()) / se)
+    z = np.array(z_all)
+    print("sweep: %d set/species/sample comparisons, KS rejections %d, median |z| %.2f, |z|>4: %.4f%%"
+          % (compared, ks_bad, np.median(z), 100 * (z > 4).mean()), flush=True)
+    assert compared > 7000
+    assert ks_bad <= 2
+    assert np.median(z) < 0.85 and (z > 4).mean() < 0.002
+    # flagged simulations (explosive sets) in equal proportion on both paths
+    n = sum(sp.n_sims for sp in specs)
+    p1, p2 = stmx["flagged"] / n, st64["flagged"] / n
+    assert abs(p1 - p2) < 5 * np.sqrt((p1 + p2) / n) + 1e-4
+
+
+def _valid_until(O, F):
+    """Per simulation, the number of leading samples that hold its state: all
+    of them unless it ran out of steps; then up to its last non-zero sample."""
+    ita = O.shape[1]
+    out = np.full(O.shape[2], ita)
+    steps = np.nonzero(F == 16)[0]  # PST_FLAG_STEPS
+    if steps.size:
+        nz = O[:, :, steps].any(axis=0)  # [ita, sims]
+        last = ita - 1 - np.argmax(nz[::-1], axis=0)
+        out[steps] = np.where(nz.any(axis=0), last + 1, 0)
+    return out
+
+
+def test_random_networks_mixed_vs_oracle(native, oracle_mod):
+    specs_all = workloads.random_networks()
+    big = max(range(len(specs_all)), key=lambda i: int(specs_all[i].tables["N"]) * int(specs_all[i].tables["M"]))
+    chosen = [big] + [i for i in range(7) if i != big]
+    specs = [specs_all[i] for i in chosen]
+    assert (int(specs[0].tables["N"]), int(specs[0].tables["M"])) == (49, 193)
+    print("case: random networks %s, mixed batch" % chosen, flush=True)
+    Om, Fm, stm = native.batch_simulate(workloads.jobs(specs, native.PRECISION_MIXED, specialize=False))
+    checks, worst = 0, 0.0
+    for i, s, O, F in zip(chosen, specs, Om, Fm):
+        kw = dict(s.kw)
+        kw["n_sims"] = 256
+        print("case: random network %d (N=%d, M=%d) oracle" % (i, s.tables["N"], s.tables["M"]), flush=True)
+        Oc, Fc, _ = oracle_mod.simulate(s.tables, s.rates, s.x0, threads=THREADS, **kw)
+        assert set(np.unique(F)) <= {0, 16} and set(np.unique(Fc)) <= {0, 16}
+        # the same share of simulations ends on the step budget
+        pa, pb = (F != 0).mean(), (Fc != 0).mean()
+        assert abs(pa - pb) < 5 * np.sqrt(0.25 / 256) + 0.02, (i, pa, pb)
+        va, vb = _valid_until(O, F), _valid_until(Oc, Fc)
+        for f in (5, 15, 30, 49):
+            a_ok, b_ok = va > f, vb > f
+            if a_ok.mean() < 0.8 or b_ok.mean() < 0.8:
+                continue
+            crit = _ks_crit(a_ok.sum(), b_ok.sum())
+            for q in range(O.shape[0]):
+                d = laws.ks2(O[q, f, a_ok], Oc[q, f, b_ok])
+                worst = max(worst, d / crit)
+                checks += 1
+                assert d < crit, (i, q, f, d, crit)
+    print("random networks: %d species/sample comparisons, worst KS / critical %.2f" % (checks, worst), flush=True)
+    assert checks > 300
