@@ -454,9 +454,11 @@ struct EngineF32 {
   }
 
   // Poisson leap with tau halving (P1_P2.py:222-259,509-540): reactions 2r and
-  // 2r+1 draw from the two halves of block (grp + r, it).  Returns 0 or a flag.
+  // 2r+1 draw from the two halves of block (grp + r, it).  tau is halved in
+  // place: the clock advances by the tau of the accepted draw
+  // (P1_P2.py:258-261).  Returns 0 or a flag.
   __device__ __forceinline__ int leap(const Ref& tab, Counters& cnt,
-                                      const float (&a)[MP], const bool (&crit)[MP], float tau,
+                                      const float (&a)[MP], const bool (&crit)[MP], float& tau,
                                       int32_t (&K)[MP], int32_t (&xn)[NP], const Words4 (&B)[(MP + 1) / 2 + 1],
                                       bool shifted) {
     constexpr int NB = (MP + 1) / 2;

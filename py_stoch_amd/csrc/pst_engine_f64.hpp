@@ -389,10 +389,11 @@ struct EngineF64 {
   }
 
   // Poisson leap with tau halving (P1_P2.py:222-259,509-540).  K holds the
-  // critical firing (if any) on entry and the accepted counts on return;
-  // returns 0 or the flag that ends the simulation.
+  // critical firing (if any) on entry and the accepted counts on return; tau
+  // is halved in place, and the clock advances by the tau of the accepted
+  // draw (P1_P2.py:258-261); returns 0 or the flag that ends the simulation.
   __device__ __forceinline__ int leap(const Ref& tab, Counters& cnt,
-                                      const double (&a)[MP], const bool (&crit)[MP], double tau,
+                                      const double (&a)[MP], const bool (&crit)[MP], double& tau,
                                       int32_t (&K)[MP], int32_t (&xn)[NP]) {
     const auto& P = tab.p();
     const int M = P.M;

@@ -363,10 +363,12 @@ struct EngineWave {
     return false;
   }
 
-  // Poisson leap with halving; x' left in L.xn.  Returns 0 or a flag.  The
+  // Poisson leap with halving; x' left in L.xn, tau halved in place (the
+  // clock advances by the tau of the accepted draw, P1_P2.py:258-261).
+  // Returns 0 or a flag.  The
   // lane's reaction slots are walked in a rolled loop with the counts in LDS,
   // so the sampler's code (and its registers) exists once.
-  __device__ int leap(const WaveJob& J, WaveLds& L, int lane, Counters& cnt, int sel, float tau, long long& ev) {
+  __device__ int leap(const WaveJob& J, WaveLds& L, int lane, Counters& cnt, int sel, float& tau, long long& ev) {
     const JobParams& P = J.p;
     const WaveTab& T = J.t;
     const int M = P.M, N = P.N;
