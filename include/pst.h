@@ -240,6 +240,29 @@ int pst_plan_autotune(pst_plan* plan, int32_t* const* O_dev, const int64_t* o_st
  * reference counterpart. */
 int pst_plan_set_launch(pst_plan* plan, int32_t lanes_shift, int32_t ssa_slice, int32_t waves_trim);
 
+/*
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  One run of one job on the library's table-driven
+ * kernel (cfg->specialize must be 0, no compiled hazards; networks up to 16
+ * species x 16 reactions), with everything the library itself needs -- the
+ * job descriptor, the work-queue counter and the counters -- in a static
+ * device array of PST_NOALLOC_SCRATCH_BYTES per device (held until the
+ * process ends) instead of memory allocated per plan.  The caller passes
+ * every input and output in device memory: rates_dev [n_sets][M] float64,
+ * x0_dev [n_sets][N] int32, I_dev [ita] float32 sample times (cfg->I is not
+ * read), O_dev [kappa][ita][o_stride] int32 and flags_dev (nullable);
+ * their values are not checked on the host.  Enqueues on `stream` and
+ * returns; with stats != NULL it waits for the run and reads the counters
+ * (kernel_ms = 0).  Results are those of pst_plan_create + pst_plan_execute
+ * with the same settings.  All calls on one device share the array: calls
+ * that overlap on one device race.
+ */
+#define PST_NOALLOC_SCRATCH_BYTES 8192
+int pst_run_noalloc(const pst_network* net, const pst_config* cfg, const double* rates_dev, const int32_t* x0_dev,
+                    int32_t n_sets, const float* I_dev, int32_t* O_dev, int64_t o_stride, int32_t* flags_dev,
+                    void* stream, pst_stats* stats);
+
 /* largest network the compiled kernels accept (species, reactions) */
 void pst_max_network(int32_t* max_species, int32_t* max_reactions);
 

@@ -121,6 +121,7 @@ class PstJob(ctypes.Structure):
 
 
 EXPORTED_SYMBOLS = (
+    "pst_run_noalloc",
     "pst_tau_leaping_run",
     "pst_gillespie_run",
     "pst_batch_run",
@@ -188,6 +189,10 @@ def load():
     L.pst_plan_set_stream.restype = ctypes.c_int
     L.pst_plan_destroy.argtypes = [ctypes.c_void_p]
     L.pst_plan_destroy.restype = None
+    L.pst_run_noalloc.argtypes = [ctypes.POINTER(PstNetwork), ctypes.POINTER(PstConfig), ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(PstStats)]
+    L.pst_run_noalloc.restype = ctypes.c_int
     L.pst_max_network.argtypes = [_P_I32, _P_I32]
     L.pst_max_network.restype = None
     L.pst_last_error.argtypes = []
@@ -272,6 +277,32 @@ def simulate(tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_F
             _ptr(flags, ctypes.c_int32) if flags is not None else None, ctypes.byref(st))
     _check(rc, fn.__name__)
     return O, (flags[: int(n_sims)] if flags is not None else None), st.as_dict()
+
+
+def run_noalloc(tables, rates_dev_ptr, x0_dev_ptr, n_sets, I_dev_ptr, O_dev_ptr, o_stride, flags_dev_ptr=None,
+                stream_ptr=None, read_stats=True, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64, n_c=10,
+                eta=0.03, ssa_burst=100, t_max=1.0, ita=None, E=(0,), n_sims=1, sims_per_set=None, seed=0,
+                sim_offset=0, max_steps=0):
+    """pst_run_noalloc: one run on the table-driven kernel with every input
+    and output in caller device memory (rates [n_sets][M] float64, x0
+    [n_sets][N] int32, sample times [ita] float32, O [kappa][ita][o_stride]
+    int32, flags) and no device allocation by the library.  Returns the
+    counters (waiting for the run) if read_stats, else None (enqueued only)."""
+    L = load()
+    N, M = int(tables["N"]), int(tables["M"])
+    if sims_per_set is None:
+        if int(n_sets) != 1:
+            raise ValueError("sims_per_set is required with several parameter rows")
+        sims_per_set = max(1, int(n_sims))
+    m = _Marshalled(tables, np.zeros((1, M)), np.zeros((1, N), np.int32), method, precision, n_c, eta, ssa_burst,
+                    t_max, np.zeros(int(ita), np.float32), E, n_sims, sims_per_set, seed, sim_offset, False,
+                    max_steps)
+    st = PstStats() if read_stats else None
+    rc = L.pst_run_noalloc(ctypes.byref(m.net), ctypes.byref(m.cfg), rates_dev_ptr, x0_dev_ptr, int(n_sets),
+                           I_dev_ptr, O_dev_ptr, int(o_stride), flags_dev_ptr, stream_ptr,
+                           ctypes.byref(st) if st is not None else None)
+    _check(rc, "pst_run_noalloc")
+    return st.as_dict() if st is not None else None
 
 
 def _autotune(h, O_ptrs, strides, flag_ptrs, stream_ptr, fraction):

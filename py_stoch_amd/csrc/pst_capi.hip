@@ -17,6 +17,7 @@
 #include <new>
 #include <vector>
 
+#include <mutex>
 #include <string>
 
 #include "../../include/pst.h"
@@ -134,8 +135,10 @@ __global__ void zero_u64(unsigned long long* p, int n) {
 }
 
 // Check a pst_network / pst_config pair and build the dense host table.
+// host_rows = false: rates, x0 and cfg->I are device arrays, and their
+// values are not checked on the host.
 int prepare(const pst_network* net, const pst_config* cfg, const double* rates, const int32_t* x0,
-            int32_t n_sets, pst::HostNet* h) {
+            int32_t n_sets, pst::HostNet* h, bool host_rows = true) {
   if (!net || !cfg || !rates || !x0) return fail(PST_EINVAL, "null network, config, rates or x0");
   const int N = net->n_species, M = net->n_reactions;
   if (N < 1 || M < 1) return fail(PST_EINVAL, "network needs at least one species and one reaction (N=%d, M=%d)", N, M);
@@ -158,7 +161,7 @@ int prepare(const pst_network* net, const pst_config* cfg, const double* rates, 
   if (!(cfg->eta > 0.0) || !std::isfinite(cfg->eta)) return fail(PST_EINVAL, "eta must be a positive finite number");
   if (cfg->method == PST_METHOD_PURE_SSA && !std::isfinite(cfg->t_max))
     return fail(PST_EINVAL, "t_max must be finite for pure SSA");
-  for (int f = 0; f < cfg->ita; ++f)
+  for (int f = 0; host_rows && f < cfg->ita; ++f)
     if (!std::isfinite(cfg->I[f])) return fail(PST_EINVAL, "sample time %d is not finite", f);
   if (net->a_size < 0 || (net->a_size > 0 && !net->A)) return fail(PST_EINVAL, "bad A table");
   if (net->v_size < 0 || (net->v_size > 0 && !net->V)) return fail(PST_EINVAL, "bad V table");
@@ -217,7 +220,7 @@ int prepare(const pst_network* net, const pst_config* cfg, const double* rates, 
     if (s < 0 || s >= N) return fail(PST_EINVAL, "recorded species %d out of range", s);
     h->E[q] = (int8_t)s;
   }
-  for (int64_t r = 0; r < need_sets; ++r) {
+  for (int64_t r = 0; host_rows && r < need_sets; ++r) {
     for (int j = 0; j < n_rates; ++j) {
       const double k = rates[r * n_rates + j];
       if (!std::isfinite(k)) return fail(PST_EINVAL, "rate %d of row %lld is not finite", j, (long long)r);
@@ -442,6 +445,63 @@ static int enqueue_launches(pst_plan* p, const int* count, int n_wave, int64_t w
     qoff += n_wave;
   }
   return PST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// pst_run_noalloc: the library's only device memory for it is this array (one
+// copy per device, held until the process ends): [0, 128) the counters,
+// [128, 132) the work-queue counter, [kNoAllocJob, kNoAllocBytes) the job
+// descriptor of one table-driven size class.  Every call on a device shares
+// its array, so calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = PST_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kNoAllocQueue = 128;
+constexpr size_t kNoAllocJob = 256;
+static_assert(pst::ST_COUNT * sizeof(unsigned long long) <= kNoAllocQueue, "counters overlap the queue");
+static_assert(sizeof(pst::JobDev<16, 16>) <= kNoAllocBytes - kNoAllocJob, "descriptor does not fit");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once per
+// device; the array's name is not a device address in host code).
+static int scratch_on_device(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(PST_ENODEV, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = (unsigned char*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return PST_OK;
+}
+
+// Writes the job descriptor (a kernel argument, so nothing is staged through
+// host memory) behind the counters and clears the counters and the queue.
+template <int NP, int MP>
+__global__ void __launch_bounds__(256) noalloc_setup(pst::JobDev<NP, MP> jd, unsigned char* base) {
+  constexpr int NW = (int)(sizeof(pst::JobDev<NP, MP>) / 4);
+  static_assert(sizeof(pst::JobDev<NP, MP>) % 4 == 0, "descriptor is not whole words");
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(&jd);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(base + kNoAllocJob);
+  for (int i = (int)threadIdx.x; i < NW; i += (int)blockDim.x) dst[i] = src[i];
+  if (threadIdx.x < kNoAllocJob / 4) reinterpret_cast<uint32_t*>(base)[threadIdx.x] = 0u;
+}
+
+template <int NP, int MP>
+static hipError_t noalloc_enqueue(const pst::HostNet& h, const pst::JobParams& P, unsigned char* base,
+                                  hipStream_t s) {
+  pst::JobDev<NP, MP> jd;
+  std::memset(&jd, 0, sizeof(jd));
+  jd.tab = pst::make_tab<NP, MP>(h);
+  jd.p = P;
+  hipLaunchKernelGGL((noalloc_setup<NP, MP>), dim3(1), dim3(256), 0, s, jd, base);
+  return hipGetLastError();
 }
 
 extern "C" {
@@ -980,6 +1040,105 @@ int pst_gillespie_run(const pst_network* net, const pst_config* cfg, const doubl
   pst_config c = *cfg;
   c.method = PST_METHOD_PURE_SSA;
   return run_host(net, &c, rates, x0, n_sets, O, flags, stats);
+}
+
+int pst_run_noalloc(const pst_network* net, const pst_config* cfg, const double* rates_dev, const int32_t* x0_dev,
+                    int32_t n_sets, const float* I_dev, int32_t* O_dev, int64_t o_stride, int32_t* flags_dev,
+                    void* stream, pst_stats* stats) {
+  g_err[0] = 0;
+  if (!net || !cfg) return fail(PST_EINVAL, "null network or config");
+  if (!I_dev) return fail(PST_EINVAL, "null sample-time array");
+  if (cfg->n_sims > 0 && !O_dev) return fail(PST_EINVAL, "null trajectory buffer");
+  if (o_stride < cfg->n_sims) return fail(PST_EINVAL, "o_stride %lld < n_sims %d", (long long)o_stride, cfg->n_sims);
+  if (net->hazards)
+    return fail(PST_EINVAL, "pst_run_noalloc runs the library's table-driven kernels: compiled hazard expressions "
+                            "need pst_plan_create");
+  if (cfg->specialize != 0)
+    return fail(PST_EINVAL, "pst_run_noalloc runs the library's table-driven kernels: specialize must be 0");
+  if (cfg->reserved != 0) return fail(PST_EINVAL, "reserved config field must be 0");
+  pst_config c = *cfg;
+  c.I = I_dev;
+  static thread_local pst::HostNet h;  // ~50 KB: kept off the stack, and no heap allocation
+  int rc = prepare(net, &c, rates_dev, x0_dev, n_sets, &h, /*host_rows=*/false);
+  if (rc != PST_OK) return rc;
+  const int cls = pst::size_class(h.N, h.M, h.kappa);
+  if (cls < 0)
+    return fail(PST_ETOOLARGE, "pst_run_noalloc: N=%d, M=%d, kappa=%d is beyond the largest table-driven size class "
+                               "(16 species, 16 reactions) that its %zu-byte device array holds",
+                h.N, h.M, h.kappa, kNoAllocBytes);
+  int cu = 0;
+  if ((rc = check_device(&cu)) != PST_OK) return rc;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  unsigned char* base = nullptr;
+  if ((rc = scratch_on_device(dev, &base)) != PST_OK) return rc;
+
+  const int simds = cu * 4;
+  int L = 64;  // simulations per wavefront, as for a plan (lanes_for)
+  while (L > 1 && ((int64_t)c.n_sims + L / 2 - 1) / (L / 2) <= (int64_t)simds) L /= 2;
+  pst::JobParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.N = h.N;
+  P.M = h.M;
+  P.kappa = c.kappa;
+  P.ita = c.ita;
+  P.n_c = c.n_c;
+  P.ssa_burst = c.ssa_burst;
+  P.method = c.method;
+  P.n_sims = c.n_sims;
+  P.sims_per_set = c.sims_per_set;
+  P.ssa_slice = kSsaSlice;
+  P.lanes = L;
+  P.n_rates = h.M;
+  P.max_steps = (uint32_t)c.max_steps;
+  P.eta = c.eta;
+  P.t_max = c.t_max;
+  P.seed = c.seed;
+  P.sim_offset = c.sim_offset;
+  P.o_stride = o_stride;
+  P.rates = rates_dev;
+  P.x0 = x0_dev;
+  P.I = I_dev;
+  P.O = O_dev;
+  P.flags = flags_dev;
+
+  hipStream_t s = (hipStream_t)stream;
+  if (cls == 0) e = noalloc_enqueue<4, 4>(h, P, base, s);
+  else if (cls == 1) e = noalloc_enqueue<8, 8>(h, P, base, s);
+  else e = noalloc_enqueue<16, 16>(h, P, base, s);
+  if (e != hipSuccess) return hip_fail(e, "descriptor kernel launch");
+  const bool mixed = c.precision == PST_PRECISION_MIXED;
+  const int bpc = mixed ? pst::blocks_per_cu_mixed(cls) : pst::blocks_per_cu_fp64(cls);
+  const int64_t need = ((int64_t)c.n_sims + L - 1) / L;
+  const int64_t resident = (int64_t)bpc * simds;
+  const int waves = (int)(need < resident ? need : resident);
+  const void* jobs = base + kNoAllocJob;
+  unsigned int* queue = (unsigned int*)(base + kNoAllocQueue);
+  unsigned long long* counters = (unsigned long long*)base;
+  rc = mixed ? pst::launch_mixed(cls, jobs, 1, queue, counters, waves, s)
+             : pst::launch_fp64(cls, jobs, 1, queue, counters, waves, s);
+  if (rc != 0) return hip_fail((hipError_t)rc, "simulation kernel launch");
+  if (stats) {
+    unsigned long long hs[pst::ST_COUNT];
+    if ((e = hipMemcpyAsync(hs, counters, sizeof(hs), hipMemcpyDeviceToHost, s)) != hipSuccess)
+      return hip_fail(e, "hipMemcpyAsync(counters)");
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "simulation kernel");
+    std::memset(stats, 0, sizeof(*stats));
+    stats->events = (int64_t)hs[pst::ST_EVENTS];
+    stats->leaps = (int64_t)hs[pst::ST_LEAPS];
+    stats->rejected_leaps = (int64_t)hs[pst::ST_REJECTED];
+    stats->ssa_steps = (int64_t)hs[pst::ST_SSA];
+    stats->flagged = (int64_t)hs[pst::ST_FLAGGED];
+    stats->uniforms = (int64_t)hs[pst::ST_UNIFORMS];
+    stats->ptrs_iterations = (int64_t)hs[pst::ST_PTRS];
+    stats->inversion_steps = (int64_t)hs[pst::ST_INV];
+    stats->leap_decisions = (int64_t)hs[pst::ST_DECISIONS];
+    stats->sample_rows = (int64_t)hs[pst::ST_SAMPLES];
+    stats->flagged_events = (int64_t)hs[pst::ST_FEVENTS];
+    stats->kernel_ms = 0.0;
+  }
+  return PST_OK;
 }
 
 }  // extern "C"

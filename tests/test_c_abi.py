@@ -107,3 +107,24 @@ def test_autotune_argument_checks(native):
 def test_set_launch_argument_checks(native):
     L = native.load()
     assert L.pst_plan_set_launch(None, 0, 7, 0) == -1  # PST_EINVAL: null plan
+
+
+def test_noalloc_argument_checks(native):
+    """pst_run_noalloc runs the table-driven kernels only and rejects what
+    they cannot take before touching a device."""
+    def err(tables, **kw):
+        a = dict(ita=11, n_sims=64, t_max=1.0, E=[0])
+        a.update(kw)
+        with pytest.raises(native.NativeError) as e:
+            native.run_noalloc(tables, 1, 1, 1, 1, 1, a["n_sims"], **a)
+        return str(e.value)
+    a = models.make_args(models.lotka_volterra(), E=[0, 1], t_max=1.0, ita=11, U=64)
+    tabs = tables_of(a)
+    big = tables_of(models.make_args(models.random_network(np.random.default_rng(1), 20, 30), E=[0]))
+    assert "beyond the largest table-driven size class" in err(big)
+    hz = dict(tabs)
+    hz["hazard_c"], hz["n_params"] = ["C(0)*X(0)", "C(1)*X(0)*X(1)", "C(2)*X(1)"], 3
+    assert "compiled hazard" in err(hz)
+    L = native.load()
+    assert L.pst_run_noalloc(None, None, None, None, 1, None, None, 0, None, None, None) == -1
+    assert b"null" in L.pst_last_error()
