@@ -71,13 +71,19 @@ def read_traffic(path=TRAFFIC_FILE):
 
 
 def work_model(tabs, kappa, st):
-    """Algorithmic VALU lane-operations of one run from the kernel's counters.
+    """Algorithmic VALU lane-operations of one mixed-precision run from the
+    kernel's counters.
 
-    Weights per counted unit (add/mul/compare/convert = 1, divide/log/exp = 4,
-    Philox4x32-10 block = 140):
-      uniform              74   half a Philox block + conversion
-      PTRS iteration       62   beyond its two uniforms
-      inversion step        3
+    Weights per counted unit (add/mul/compare/convert = 1, divide/log/exp/
+    sqrt = 4, Philox4x32-10 block = 140):
+      uniform              37   one Philox word (a quarter block) + conversion
+      PTRS iteration       62   beyond its two uniforms (means >= 2^16)
+      inversion step        3   sequential search (means < 10)
+      asymptotic inversion 95   normal quantile (log, sqrt, 10-term
+                                polynomial), rsqrt, three polynomials of
+                                degree 10/10/6, floor, band test
+      FP64 decision       300   pmf, log/exp/ln Gamma and the first terms of
+                                the CDF sum (rare)
       leap decision        P + 2M + 23 nnz(V) + 5   propensities, a0, critical
                                                    test, mu/sigma^2, Cao bound
       accepted leap        2 nnz(V) + N + 1         state update and checks
@@ -91,7 +97,8 @@ def work_model(tabs, kappa, st):
     N, M = int(tabs["N"]), int(tabs["M"])
     nnz = int(np.asarray(tabs["V"]).reshape(-1, 4).shape[0])
     P = int(sum(1 + 2 * int(po[:, j].sum()) for j in range(M)))
-    w = (74 * st["uniforms"] + 62 * st["ptrs_iterations"] + 3 * st["inversion_steps"]
+    w = (37 * st["uniforms"] + 62 * st["ptrs_iterations"] + 3 * st["inversion_steps"]
+         + 95 * st["asym_draws"] + 300 * st["exact_draws"]
          + (P + 2 * M + 23 * nnz + 5) * st["leap_decisions"]
          + (2 * nnz + N + 1) * st["leaps"] + (2 * nnz + N) * st["rejected_leaps"]
          + (P + 4 * M + 13 + nnz / M) * st["ssa_steps"] + 8 * kappa * st["sample_rows"])
@@ -225,9 +232,36 @@ def valid_events(st):
     return st["events"] - st["flagged_events"]
 
 
-def cpu_baseline(tabs, rates, x0, I, E, budget_s=15.0):
+def host_threads():
+    """Cores this process may run on (its CPU affinity), at most the
+    OMP_NUM_THREADS share when one is set (the GPU box grants a GPU job 16)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_model_name():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def cpu_baseline(tabs, rates, x0, I, E, budget_s=8.0, repeats=3):
+    """The FP64 oracle on the host cores: the first n simulations of the same
+    workload, n sized so one run takes about budget_s; the median of
+    `repeats` runs."""
     from oracle import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     probe = 256
     t0 = time.perf_counter()
     oracle.simulate(tabs, rates, x0, method=0, t_max=T_END, I=I, E=E, n_sims=probe, seed=SEED,
@@ -235,22 +269,19 @@ def cpu_baseline(tabs, rates, x0, I, E, budget_s=15.0):
     dt = time.perf_counter() - t0
     n = int(min(U, max(probe, (budget_s / max(dt, 1e-6)) * probe)))
     n = max(threads, (n // threads) * threads)
-    t0 = time.perf_counter()
-    _, _, st = oracle.simulate(tabs, rates, x0, method=0, t_max=T_END, I=I, E=E, n_sims=n, seed=SEED,
-                               threads=threads)
-    dt = time.perf_counter() - t0
-    cpu_model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    return {"value": valid_events(st) / dt, "unit": "reaction events/s", "cores": threads, "kind": "port",
-            "sample": "first %d of %d simulations of the same workload, FP64 oracle (C, OpenMP), "
-                      "%.2f s, %s" % (n, U, dt, cpu_model)}
+    rates_s = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        _, _, st = oracle.simulate(tabs, rates, x0, method=0, t_max=T_END, I=I, E=E, n_sims=n, seed=SEED,
+                                   threads=threads)
+        dt = time.perf_counter() - t0
+        rates_s.append((valid_events(st) / dt, dt))
+    rates_s.sort()
+    value, dt = rates_s[len(rates_s) // 2]
+    return {"value": value, "unit": "reaction events/s", "cores": threads, "kind": "port",
+            "sample": "first %d of %d simulations of the same workload, FP64 oracle (C, OpenMP, %d threads), "
+                      "median of %d runs (%.2f s each), %s; host reports %d CPUs"
+                      % (n, U, threads, repeats, dt, cpu_model_name(), os.cpu_count() or 0)}
 
 
 def _time_batch(plan, O_list, K, W, stream):
@@ -403,7 +434,7 @@ def random_networks_workload(args, n_check=8, check_sims=512):
     torch.cuda.synchronize()
     print("bench: random networks, oracle check", file=sys.stderr, flush=True)
     zmax, flag_frac = [], []
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     t_cpu, ev_cpu = 0.0, 0
     for s, O, F in list(zip(specs, O_list, F_list))[:n_check]:
         kw = dict(s.kw)

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define PST_ABI_VERSION 4
+#define PST_ABI_VERSION 5
 
 /* status codes */
 #define PST_OK 0
@@ -132,6 +132,10 @@ typedef struct pst_stats {
   int64_t leap_decisions;     /* tau-leap decisions taken (P1_P2 bodies)      */
   int64_t sample_rows;        /* sample rows written (per recorded row of O)  */
   int64_t flagged_events;     /* the part of events from flagged simulations  */
+  int64_t asym_draws;         /* mixed precision: Poisson draws by the asymptotic
+                                 inversion (10 <= mean < 2^16)                */
+  int64_t exact_draws;        /* of those, decided in FP64 near an integer
+                                 boundary of the inverse CDF, or in a far tail */
   double kernel_ms;           /* device time of the simulation kernel         */
 } pst_stats;
 
@@ -262,6 +266,19 @@ int pst_plan_set_launch(pst_plan* plan, int32_t lanes_shift, int32_t ssa_slice, 
 int pst_run_noalloc(const pst_network* net, const pst_config* cfg, const double* rates_dev, const int32_t* x0_dev,
                     int32_t n_sets, const float* I_dev, int32_t* O_dev, int64_t o_stride, int32_t* flags_dev,
                     void* stream, pst_stats* stats);
+
+/*
+ * No counterpart in the reference: a validation hook.  Runs the Poisson
+ * sampler of the mixed-precision kernels (the replacement of curand_poisson,
+ * P1_P2.py:239) on n given means lam[i] and Philox words words[i] (host
+ * arrays; the uniform is ((w >> 8) + 1) 2^-24) and returns the counts k[i]
+ * and the branch each draw took, path[i]: 0 mean <= 0, 1 sequential-search
+ * inversion (mean < 10), 2 asymptotic inversion (10 <= mean < 2^16), 3 the
+ * same decided exactly in FP64, 4 PTRS (mean >= 2^16, on the stream of
+ * simulation i, seed 0).  Paths 1-3 are inversions: k[i] is the smallest n
+ * with P(X <= n) >= u.  n <= 2^28.
+ */
+int pst_poisson_probe_mixed(const float* lam, const uint32_t* words, int64_t n, int32_t* k, int32_t* path);
 
 /* largest network the compiled kernels accept (species, reactions) */
 void pst_max_network(int32_t* max_species, int32_t* max_reactions);

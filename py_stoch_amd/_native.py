@@ -12,7 +12,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libpst_hip.so")
-ABI_VERSION = 4  # PST_ABI_VERSION of include/pst.h
+ABI_VERSION = 5  # PST_ABI_VERSION of include/pst.h
 
 METHOD_TAU_LEAPING = 0
 METHOD_FORCED_SSA = 1
@@ -86,6 +86,8 @@ class PstStats(ctypes.Structure):
         ("leap_decisions", ctypes.c_int64),
         ("sample_rows", ctypes.c_int64),
         ("flagged_events", ctypes.c_int64),
+        ("asym_draws", ctypes.c_int64),
+        ("exact_draws", ctypes.c_int64),
         ("kernel_ms", ctypes.c_double),
     ]
 
@@ -121,6 +123,7 @@ class PstJob(ctypes.Structure):
 
 
 EXPORTED_SYMBOLS = (
+    "pst_poisson_probe_mixed",
     "pst_run_noalloc",
     "pst_tau_leaping_run",
     "pst_gillespie_run",
@@ -193,6 +196,9 @@ def load():
                                   ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(PstStats)]
     L.pst_run_noalloc.restype = ctypes.c_int
+    L.pst_poisson_probe_mixed.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint32),
+                                          ctypes.c_int64, _P_I32, _P_I32]
+    L.pst_poisson_probe_mixed.restype = ctypes.c_int
     L.pst_max_network.argtypes = [_P_I32, _P_I32]
     L.pst_max_network.restype = None
     L.pst_last_error.argtypes = []
@@ -303,6 +309,22 @@ def run_noalloc(tables, rates_dev_ptr, x0_dev_ptr, n_sets, I_dev_ptr, O_dev_ptr,
                            ctypes.byref(st) if st is not None else None)
     _check(rc, "pst_run_noalloc")
     return st.as_dict() if st is not None else None
+
+
+def poisson_probe_mixed(lam, words):
+    """The mixed-precision kernels' Poisson sampler on given means (float32)
+    and Philox words (uint32): returns (counts, branch per draw) -- see
+    pst_poisson_probe_mixed in include/pst.h."""
+    lam = np.ascontiguousarray(lam, np.float32).reshape(-1)
+    words = np.ascontiguousarray(words, np.uint32).reshape(-1)
+    if lam.shape != words.shape:
+        raise ValueError("lam and words differ in length")
+    k = np.empty(lam.shape, np.int32)
+    path = np.empty(lam.shape, np.int32)
+    rc = load().pst_poisson_probe_mixed(_ptr(lam, ctypes.c_float), _ptr(words, ctypes.c_uint32), lam.size,
+                                        _ptr(k, ctypes.c_int32), _ptr(path, ctypes.c_int32))
+    _check(rc, "pst_poisson_probe_mixed")
+    return k, path
 
 
 def _autotune(h, O_ptrs, strides, flag_ptrs, stream_ptr, fraction):

@@ -28,7 +28,7 @@ struct NetTab {
 
 enum StatIdx {
   ST_EVENTS = 0, ST_LEAPS, ST_REJECTED, ST_SSA, ST_FLAGGED,
-  ST_UNIFORMS, ST_PTRS, ST_INV, ST_DECISIONS, ST_SAMPLES, ST_FEVENTS,
+  ST_UNIFORMS, ST_PTRS, ST_INV, ST_DECISIONS, ST_SAMPLES, ST_FEVENTS, ST_ASYM, ST_EXACT,
   ST_COUNT = 16
 };
 
@@ -221,6 +221,9 @@ int launch_fp64(int cls, const void* jobs, int n_jobs, unsigned int* queue, unsi
                 int waves, hipStream_t stream);
 int launch_mixed(int cls, const void* jobs, int n_jobs, unsigned int* queue, unsigned long long* stats,
                  int waves, hipStream_t stream);
+// the mixed-precision Poisson sampler on given means and words (validation)
+int launch_poisson_probe(const float* lam, const uint32_t* words, int64_t n, int32_t* k, int32_t* path,
+                         hipStream_t stream);
 // resident 256-lane blocks per CU for a class (occupancy)
 int blocks_per_cu_fp64(int cls);
 int blocks_per_cu_mixed(int cls);

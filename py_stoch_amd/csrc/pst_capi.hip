@@ -875,6 +875,8 @@ int pst_plan_read_stats(pst_plan* p, pst_stats* st) {
   st->leap_decisions = (int64_t)h[pst::ST_DECISIONS];
   st->sample_rows = (int64_t)h[pst::ST_SAMPLES];
   st->flagged_events = (int64_t)h[pst::ST_FEVENTS];
+  st->asym_draws = (int64_t)h[pst::ST_ASYM];
+  st->exact_draws = (int64_t)h[pst::ST_EXACT];
   st->kernel_ms = (double)ms;
   return PST_OK;
 }
@@ -1136,9 +1138,42 @@ int pst_run_noalloc(const pst_network* net, const pst_config* cfg, const double*
     stats->leap_decisions = (int64_t)hs[pst::ST_DECISIONS];
     stats->sample_rows = (int64_t)hs[pst::ST_SAMPLES];
     stats->flagged_events = (int64_t)hs[pst::ST_FEVENTS];
+    stats->asym_draws = (int64_t)hs[pst::ST_ASYM];
+    stats->exact_draws = (int64_t)hs[pst::ST_EXACT];
     stats->kernel_ms = 0.0;
   }
   return PST_OK;
+}
+
+int pst_poisson_probe_mixed(const float* lam, const uint32_t* words, int64_t n, int32_t* k, int32_t* path) {
+  g_err[0] = 0;
+  if (n < 0 || (n > 0 && (!lam || !words || !k || !path))) return fail(PST_EINVAL, "null array or negative count");
+  if (n > ((int64_t)1 << 28)) return fail(PST_EINVAL, "at most 2^28 draws per call");
+  if (n == 0) return PST_OK;
+  int cu = 0;
+  int rc = check_device(&cu);
+  if (rc != PST_OK) return rc;
+  float* d_lam = nullptr;
+  uint32_t* d_w = nullptr;
+  int32_t *d_k = nullptr, *d_p = nullptr;
+  hipError_t e;
+  if ((e = hipMalloc(&d_lam, n * sizeof(float))) != hipSuccess || (e = hipMalloc(&d_w, n * sizeof(uint32_t))) != hipSuccess ||
+      (e = hipMalloc(&d_k, n * sizeof(int32_t))) != hipSuccess || (e = hipMalloc(&d_p, n * sizeof(int32_t))) != hipSuccess) {
+    rc = hip_fail(e, "hipMalloc(probe)");
+  } else if ((e = hipMemcpy(d_lam, lam, n * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+             (e = hipMemcpy(d_w, words, n * sizeof(uint32_t), hipMemcpyHostToDevice)) != hipSuccess) {
+    rc = hip_fail(e, "hipMemcpy(probe)");
+  } else if ((rc = pst::launch_poisson_probe(d_lam, d_w, n, d_k, d_p, nullptr)) != 0) {
+    rc = hip_fail((hipError_t)rc, "probe kernel launch");
+  } else if ((e = hipMemcpy(k, d_k, n * sizeof(int32_t), hipMemcpyDeviceToHost)) != hipSuccess ||
+             (e = hipMemcpy(path, d_p, n * sizeof(int32_t), hipMemcpyDeviceToHost)) != hipSuccess) {
+    rc = hip_fail(e, "probe kernel");
+  }
+  if (d_lam) (void)hipFree(d_lam);
+  if (d_w) (void)hipFree(d_w);
+  if (d_k) (void)hipFree(d_k);
+  if (d_p) (void)hipFree(d_p);
+  return rc;
 }
 
 }  // extern "C"

@@ -82,6 +82,8 @@ struct Work {
   unsigned int inv;        // inversion search steps
   unsigned int decisions;  // tau-leap decisions (P1_P2 bodies with a0 > 0)
   unsigned int samples;    // sample rows written
+  unsigned int asym;       // Poisson draws by the asymptotic inversion (mixed precision)
+  unsigned int exact;      // of those, decided in FP64 near an integer boundary or in a far tail
 };
 
 // Per-simulation stream key.  Every draw comes from the Philox block
@@ -223,5 +225,112 @@ __device__ __forceinline__ int32_t rn_f32_to_int(float f) {
 __device__ __forceinline__ float fast_log(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
 __device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// ---------------------------------------------------------------------------
+// Poisson inversion of the mixed-precision kernels for POIS_ASYM_MIN <=
+// lambda < POIS_ASYM_MAX: N = min{n : P(X <= n) >= u} for one uniform u.
+// P(X <= n) = Q(n + 1, lambda) (regularized upper incomplete gamma), so N =
+// ceil(x* - 1) for the root x* of Q(x*, lambda) = u.  Temme's uniform
+// asymptotic expansion of Q (DLMF 8.12.3-8.12.8) solved for the shape gives
+//   x* = lambda r0(w) + delta0(w) + d1(w) / lambda + O(lambda^-2),
+//   s = Phi^{-1}(u), w = s / sqrt(lambda), f(r0) = w,
+//   f(r) = sign(r - 1) sqrt(2 (1 - r + r ln r)),
+// with r0 - 1 = w QW(w), delta0 = D0(w), d1 = D1(w) as polynomials on the
+// central range -0.7 <= w <= 1.0 (poisson_coefs.py derives them and states
+// the formulas).  Phi^{-1}(u) = sqrt(2) erfinv(2u - 1) with erfinv(x) / x a
+// polynomial in -ln(1 - x^2) (EC) or its square root (ET).
+// Error of the FP32 evaluation against the exact root, measured on 1.2e5
+// draws over 10 <= lambda <= 2^16: below 0.006 / lambda^2 (the truncated
+// expansion) plus 25 sqrt(lambda) 2^-24 (FP32 rounding, at |s| ~ 5); the
+// band POIS_EPS = 0.025 / lambda^2 + 20 (1 + |s|) sqrt(lambda) 2^-24 is at
+// least 4.5x that everywhere measured.  A draw whose x* lies within the band
+// of an integer, or whose w is outside the central range (lambda < 60 only,
+// u in a far tail), is decided exactly in FP64 (poisson_exact_f64): so N is
+// the exact inverse CDF at u, not an approximation of it.
+// ---------------------------------------------------------------------------
+constexpr float POIS_ASYM_MIN = 10.0f;
+constexpr float POIS_ASYM_MAX = 65536.0f;
+__device__ constexpr float POIS_QW[11] = {1.0f, 0.166666672f, -0.0138887819f, 0.00370368059f, -0.0013325602f, 0.000559360196f, -0.000250508776f, 0.000120938115f, -7.9569676e-05f, 5.17265544e-05f, -1.57486393e-05f};
+__device__ constexpr float POIS_D0[11] = {0.333333343f, -0.0277777705f, 0.00864164345f, -0.00352370599f, 0.00165824813f, -0.000848049065f, 0.000435102294f, -0.000240344336f, 0.000193511994f, -0.000140543721f, 4.42704841e-05f};
+__device__ constexpr float POIS_D1[7] = {-0.0197603144f, 0.017301308f, -0.0124651166f, 0.00835086685f, -0.00733779557f, 0.00694462517f, -0.00295915455f};
+__device__ constexpr float POIS_EC[10] = {1.50140941f, 0.24664028f, -0.00417738827f, -0.00125272525f, 0.0002183459f, -4.97306564e-06f, -3.46309662e-06f, 4.68265824e-07f, 2.32651161e-08f, -8.9413108e-09f};
+__device__ constexpr float POIS_ET[10] = {2.83297682f, 1.00167501f, 0.0094453292f, -0.00763711287f, 0.00568660861f, -0.00359788747f, 0.00148612761f, -5.66016934e-05f, -0.000310500182f, 0.000116065268f};
+
+template <int NC>
+__device__ __forceinline__ float horner_f32(const float (&c)[NC], float z) {
+  float acc = c[NC - 1];
+#pragma unroll
+  for (int i = NC - 2; i >= 0; --i) acc = __builtin_fmaf(acc, z, c[i]);
+  return acc;
+}
+
+// Phi^{-1}(u) for u in (0, 1]; u = 1 is taken as 1 - 2^-25
+__device__ __forceinline__ float normal_quantile_f32(float u) {
+  u = __builtin_fminf(u, 1.0f - 0x1p-25f);
+  const float x = 2.0f * u - 1.0f;                        // exact for u = k 2^-24
+  const float lw = -fast_log(4.0f * (u * (1.0f - u)));   // -ln(1 - x^2), 1 - u exact
+  const float pc = horner_f32(POIS_EC, lw - 2.5f);
+  const float pt = horner_f32(POIS_ET, __builtin_amdgcn_sqrtf(lw) - 3.0f);
+  return 1.41421356f * ((lw < 5.0f) ? pc : pt) * x;
+}
+
+// The FP32 part: returns the candidate N and sets `exact` when the draw must
+// be decided in FP64, with `m` the integer boundary between the two
+// candidates m - 1 and m (central range), or m = -1 (outside it).
+__device__ __forceinline__ int32_t poisson_asym_f32(float lam, float u, bool& exact, int32_t& m) {
+  const float s = normal_quantile_f32(u);
+  const float rs = fast_rsq(lam);
+  const float w = s * rs;
+  const float sl = lam * rs;  // sqrt(lambda)
+  const float L = __builtin_floorf(lam);
+  const float rl = fast_rcp(lam);
+  const float y = ((lam - L) + (s * sl) * horner_f32(POIS_QW, w)) + (horner_f32(POIS_D0, w) + horner_f32(POIS_D1, w) * rl);
+  const float fy = __builtin_floorf(y);
+  const float fr = y - fy;
+  const float eps = 0.025f * rl * rl + (20.0f * 0x1p-24f) * (1.0f + __builtin_fabsf(s)) * sl;
+  const bool central = (w >= -0.7f) & (w <= 1.0f);
+  exact = !central || (fr < eps) || (fr > 1.0f - eps);
+  m = central ? (int32_t)L + (int32_t)fy + (fr > 0.5f ? 1 : 0) : -1;
+  return (int32_t)L + (int32_t)fy;
+}
+
+// Exact FP64 decision for a draw flagged by poisson_asym_f32: with m >= 0,
+// N is m - 1 if P(X <= m - 1) >= u and m otherwise (the FP32 root is within
+// the band of m); the CDF is a sum of pmf terms from p(m - 1) downwards, or
+// 1 minus a sum from p(m) upwards above the mean, to 1e-17 of the sum.  With
+// m = -1 (lambda < 60, u in a far tail) a search from n = 0.
+__device__ __noinline__ int32_t poisson_exact_f64(double lam, double u, int32_t m) {
+  if (m < 0) {
+    double p = exp_f64(-lam), S = p;
+    int32_t k = 0;
+    while (u > S && k < 100000) {
+      k += 1;
+      p = p * lam / (double)k;
+      S += p;
+    }
+    return k;
+  }
+  if (m == 0) return 0;
+  const double ll = log_f64(lam);
+  if ((double)(m - 1) <= lam) {
+    // P(X <= m - 1) = sum_{k <= m - 1} p(k), from p(m - 1) down
+    double t = exp_f64(-lam + (double)(m - 1) * ll - lgamma_int_f64((double)m));
+    double S = t;
+    for (int32_t k = m - 1; k > 0 && t > 1e-17 * S; --k) {
+      t = t * (double)k / lam;
+      S += t;
+    }
+    return (u <= S) ? m - 1 : m;
+  }
+  // P(X >= m) = sum_{k >= m} p(k), from p(m) up; P(X <= m - 1) = 1 - that
+  double t = exp_f64(-lam + (double)m * ll - lgamma_int_f64((double)m + 1.0));
+  double T = t;
+  for (int32_t k = m + 1; t > 1e-17 * T; ++k) {
+    t = t * lam / (double)k;
+    T += t;
+  }
+  return (u <= 1.0 - T) ? m - 1 : m;
+}
 
 }  // namespace pst

@@ -157,12 +157,12 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
     }
   }
 
-  const unsigned long long vals[ST_FEVENTS + 1] = {cnt.events, cnt.leaps, cnt.rejected, cnt.ssa, cnt.flagged,
-                                                   cnt.w.uniforms, cnt.w.ptrs, cnt.w.inv, cnt.w.decisions,
-                                                   cnt.w.samples, cnt.fevents};
+  const unsigned long long vals[ST_EXACT + 1] = {cnt.events, cnt.leaps, cnt.rejected, cnt.ssa, cnt.flagged,
+                                                 cnt.w.uniforms, cnt.w.ptrs, cnt.w.inv, cnt.w.decisions,
+                                                 cnt.w.samples, cnt.fevents, cnt.w.asym, cnt.w.exact};
   // (32-bit per-lane counters: a lane's share of one launch stays far below 2^32)
 #pragma unroll
-  for (int i = 0; i <= ST_FEVENTS; ++i) {
+  for (int i = 0; i <= ST_EXACT; ++i) {
     const unsigned long long sum = wave_sum(vals[i]);
     if (lane == 0) atomicAdd(&stats[i], sum);
   }

@@ -52,15 +52,6 @@ __device__ __forceinline__ float calc_g_f32(int h, int ht, int xi) {
   return 0.0f;
 }
 
-// x[idx] for a run-time idx in [0, NP) (floats), masked like pick_i
-template <int NP>
-__device__ __forceinline__ float pick_m(const float (&x)[NP], int idx) {
-  uint32_t r = 0;
-#pragma unroll
-  for (int i = 0; i < NP; ++i) r |= __builtin_bit_cast(uint32_t, x[i]) & (0u - (uint32_t)(idx == i));
-  return __builtin_bit_cast(float, r);
-}
-
 // log(1 + d) with the rounding of 1 + d corrected (Goldberg's method)
 __device__ __forceinline__ float log1p_f32(float d) {
   const float u = 1.0f + d;
@@ -114,50 +105,48 @@ __device__ __forceinline__ bool ptrs_slow_f32(const PtrsF32& P, float V, float k
   return lhs <= log_pmf_f32(kf, P.lam);
 }
 
-// PTRS iterations it0, it0 + 1, ... of the draw whose iteration `it` uses the
-// words of half h of block (c, it); -1 at the iteration limit.
-__device__ __noinline__ Draw ptrs_rest_f32(uint32_t k0, uint32_t k1, uint32_t s_lo, uint32_t s_hi, uint64_t c, int h,
-                                           int it0, float lam) {
-  Key key;
-  key.k0 = k0;
-  key.k1 = k1;
-  key.s_lo = s_lo;
-  key.s_hi = s_hi;
-  const PtrsF32 P = ptrs_setup_f32(lam);
-  Draw d = {-1, 0u};
-  for (int it = it0; it < PTRS_MAX_ITERS; ++it) {
-    const Words4 bw = key.block(c, (uint32_t)it);
-    d.iters += 1;
-    const float U = u24(h ? bw.w2 : bw.w0) - 0.5f;
-    const float V = u24(h ? bw.w3 : bw.w1);
-    float kf, us;
-    if (ptrs_fast_f32(P, U, V, kf, us) || ptrs_slow_f32(P, V, kf, us)) {
-      d.k = (int64_t)kf;
-      break;
-    }
-  }
-  return d;
-}
+// PTRS for Poisson means of 2^16 and above (rare: the inversion of
+// poisson_asym_f32 covers the means below): iteration i of the draw in word
+// slot `slot` of block c uses block (c, 1 + 4 i + slot), U from its word 0
+// and V from word 1, so no two draws share a block.  FP32 below 2^22, FP64
+// above (FP32 cannot resolve the integers there).  k = -1 at the iteration
+// limit.
+constexpr float BIG_LAMBDA_F32 = 4194304.0f;  // 2^22
 
-// Poisson means too large for FP32 integer resolution: PTRS in FP64
-__device__ __noinline__ Draw ptrs_big_f64(uint32_t k0, uint32_t k1, uint32_t s_lo, uint32_t s_hi, uint64_t c, int h,
-                                          double lam) {
+__device__ __noinline__ Draw ptrs_wide(uint32_t k0, uint32_t k1, uint32_t s_lo, uint32_t s_hi, uint64_t c, int slot,
+                                       float lamf) {
   Key key;
   key.k0 = k0;
   key.k1 = k1;
   key.s_lo = s_lo;
   key.s_hi = s_hi;
+  Draw d = {-1, 0u};
+  if (lamf < BIG_LAMBDA_F32) {
+    const PtrsF32 P = ptrs_setup_f32(lamf);
+    for (int i = 0; i < PTRS_MAX_ITERS; ++i) {
+      const Words4 bw = key.block(c, 1u + 4u * (uint32_t)i + (uint32_t)slot);
+      d.iters += 1;
+      const float U = u24(bw.w0) - 0.5f;
+      const float V = u24(bw.w1);
+      float kf, us;
+      if (ptrs_fast_f32(P, U, V, kf, us) || ptrs_slow_f32(P, V, kf, us)) {
+        d.k = (int64_t)kf;
+        break;
+      }
+    }
+    return d;
+  }
+  const double lam = (double)lamf;
   const double loglam = log_f64(lam);
   const double b = 0.931 + 2.53 * __builtin_sqrt(lam);
   const double a = -0.059 + 0.02483 * b;
   const double vr = 0.9277 - 3.6224 / (b - 2.0);
   const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
-  Draw d = {-1, 0u};
-  for (int it = 0; it < PTRS_MAX_ITERS; ++it) {
-    const Words4 bw = key.block(c, (uint32_t)it);
+  for (int i = 0; i < PTRS_MAX_ITERS; ++i) {
+    const Words4 bw = key.block(c, 1u + 4u * (uint32_t)i + (uint32_t)slot);
     d.iters += 1;
-    const double U = u53(h ? bw.w2 : bw.w0, h ? bw.w3 : bw.w1) - 0.5;
-    const double V = u53(h ? bw.w3 : bw.w1, h ? bw.w2 : bw.w0);
+    const double U = u53(bw.w0, bw.w1) - 0.5;
+    const double V = u53(bw.w1, bw.w0);
     const double us = 0.5 - __builtin_fabs(U);
     const double kf = __builtin_floor((2.0 * a / us + b) * U + lam + 0.43);
     if (us >= 0.07 && V <= vr) {
@@ -174,8 +163,6 @@ __device__ __noinline__ Draw ptrs_big_f64(uint32_t k0, uint32_t k1, uint32_t s_l
   }
   return d;
 }
-
-constexpr float BIG_LAMBDA_F32 = 4194304.0f;  // 2^22
 
 // Per-lane state and steps of one mixed-precision simulation.
 template <class Ref>
@@ -249,7 +236,7 @@ struct EngineF32 {
     // blocks after it, or from bc when there is no one-block draw; block
     // bc + NB is generated at the leap, only if a lane of the wavefront made
     // a one-block draw.
-    constexpr int NB = (MP + 1) / 2;
+    constexpr int NB = (MP + 3) / 4;
     const uint64_t bc0 = bc;
     Words4 B[NB + 1];
     B[0] = key.block(bc, 0);
@@ -257,7 +244,7 @@ struct EngineF32 {
     if (!SSA_ONLY) {
 #pragma unroll
       for (int r = 1; r < NB; ++r)
-        if (2 * (r - 1) < M) B[r] = key.block(bc + (uint64_t)r, 0);
+        if (4 * (r - 1) < M) B[r] = key.block(bc + (uint64_t)r, 0);
     }
     float a[MP];
     propensities_f32<Ref>(tab, k, x, a);
@@ -453,18 +440,23 @@ struct EngineF32 {
     return MODE_DECIDE;
   }
 
-  // Poisson leap with tau halving (P1_P2.py:222-259,509-540): reactions 2r and
-  // 2r+1 draw from the two halves of block (grp + r, it).  tau is halved in
-  // place: the clock advances by the tau of the accepted draw
-  // (P1_P2.py:258-261).  Returns 0 or a flag.
+  // Poisson leap with tau halving (P1_P2.py:222-259,509-540).  Every draw is
+  // an inversion of the Poisson CDF at one 24-bit uniform: reaction j reads
+  // word j % 4 of block grp + j / 4 (so one block serves four reactions);
+  // lambda < 10 by sequential search, 10 <= lambda < 2^16 by the asymptotic
+  // inversion with exact FP64 decisions near integer boundaries (both are
+  // inversions, so no draw is rejected and repeated); lambda >= 2^16 by
+  // PTRS on blocks of its own (ptrs_wide).  tau is halved in place: the clock
+  // advances by the tau of the accepted draw (P1_P2.py:258-261).  Returns 0
+  // or a flag.
   __device__ __forceinline__ int leap(const Ref& tab, Counters& cnt,
                                       const float (&a)[MP], const bool (&crit)[MP], float& tau,
-                                      int32_t (&K)[MP], int32_t (&xn)[NP], const Words4 (&B)[(MP + 1) / 2 + 1],
+                                      int32_t (&K)[MP], int32_t (&xn)[NP], const Words4 (&B)[(MP + 3) / 4 + 1],
                                       bool shifted) {
-    constexpr int NB = (MP + 1) / 2;
+    constexpr int NB = (MP + 3) / 4;
     const auto& P = tab.p();
     const int M = P.M;
-    const int nblk = (M + 1) >> 1;
+    const int nblk = (M + 3) >> 2;
     for (int h = 0; h < MAX_HALVINGS; ++h) {
       const uint64_t grp = bc;
       bc += (uint64_t)nblk;
@@ -472,7 +464,7 @@ struct EngineF32 {
       Words4 W[NB];
 #pragma unroll
       for (int r = 0; r < NB; ++r) {
-        if (2 * r < M) {
+        if (4 * r < M) {
           if (h == 0) {
             W[r].w0 = shifted ? B[r + 1].w0 : B[r].w0;
             W[r].w1 = shifted ? B[r + 1].w1 : B[r].w1;
@@ -483,51 +475,55 @@ struct EngineF32 {
           }
         }
       }
-      float lam[MP];
-      unsigned pend = 0, inv = 0, drawn = 0;
+      float lam[MP], u[MP];
+      unsigned inv = 0, exact = 0, wide = 0, nasym = 0;
+      int32_t mb[MP];
       bool bad = false;
 #pragma unroll
       for (int j = 0; j < MP; ++j) {
         const float l = (j < M && !crit[j]) ? a[j] * tau : 0.0f;
         lam[j] = l;
-        const uint32_t wu = (j & 1) ? W[j >> 1].w2 : W[j >> 1].w0;
-        const uint32_t wv = (j & 1) ? W[j >> 1].w3 : W[j >> 1].w1;
-        const bool use = (l >= 10.0f) & (l < BIG_LAMBDA_F32);
-        bool ok = false;
-        float kf = 0.0f;
-        if (wave_any(use)) {
-          // PTRS iteration 0 (squeeze, then the log-density test)
-          const PtrsF32 Pt = ptrs_setup_f32(fmaxf(l, 10.0f));
-          float us;
-          const float V = u24(wv);
-          ok = ptrs_fast_f32(Pt, u24(wu) - 0.5f, V, kf, us);
-          if (use & !ok) ok = ptrs_slow_f32(Pt, V, kf, us);
-          ok = ok & use;
-        }
-        if (!crit[j]) K[j] = ok ? (int32_t)kf : 0;
-        pend |= ((l >= 10.0f) & !ok) ? (1u << j) : 0u;
-        drawn |= use ? (1u << j) : 0u;
-        inv |= ((l > 0.0f) & (l < 10.0f)) ? (1u << j) : 0u;
+        const Words4& wd = W[j >> 2];
+        const uint32_t word = (j & 3) == 0 ? wd.w0 : (j & 3) == 1 ? wd.w1 : (j & 3) == 2 ? wd.w2 : wd.w3;
+        u[j] = u24(word);
+        const bool asym = (l >= POIS_ASYM_MIN) & (l < POIS_ASYM_MAX);
+        bool ex = false;
+        int32_t m = -1;
+        int32_t k = 0;
+        if (wave_any(asym)) k = poisson_asym_f32(__builtin_fmaxf(l, POIS_ASYM_MIN), u[j], ex, m);
+        if (!crit[j]) K[j] = asym ? k : 0;
+        mb[j] = m;
+        exact |= (asym & ex) ? (1u << j) : 0u;
+        nasym += asym ? 1u : 0u;
+        inv |= ((l > 0.0f) & (l < POIS_ASYM_MIN)) ? (1u << j) : 0u;
+        wide |= (l >= POIS_ASYM_MAX) ? (1u << j) : 0u;
         bad = bad | (l >= (float)LAMBDA_LIMIT);
       }
-      cnt.w.ptrs += (unsigned int)__builtin_popcount(drawn);
-      cnt.w.uniforms += 2u * (unsigned int)__builtin_popcount(drawn);
       if (bad) return FLAG_OVERFLOW;
+      cnt.w.asym += nasym;
+      cnt.w.uniforms += nasym;
+      // near an integer boundary, or in a far tail: decide exactly in FP64
+      if (wave_any(exact != 0u)) {
+        cnt.w.exact += (unsigned int)__builtin_popcount(exact);
+#pragma unroll
+        for (int j = 0; j < MP; ++j)
+          if ((exact >> j) & 1u) K[j] = poisson_exact_f64((double)lam[j], (double)u[j], mb[j]);
+      }
       // sequential-search inversion (0 < lambda < 10) for all such reactions
       // in lock-step, two search steps per trip: a live search at k = it
       // steps to it + 1 and it + 2 and stops at the first k with u <= F(k);
       // p and F past a search's end are computed but never read
-      if (inv != 0u) {
-        float p[MP], Fc[MP], u[MP];
+      if (wave_any(inv != 0u)) {
+        float p[MP], Fc[MP];
         bool live[MP];
         bool any_live = false;
 #pragma unroll
         for (int j = 0; j < MP; ++j) {
           const bool on = (inv >> j) & 1u;
-          u[j] = u24((j & 1) ? W[j >> 1].w2 : W[j >> 1].w0);
           p[j] = fast_exp(-lam[j]);
           Fc[j] = p[j];
           live[j] = on && u[j] > Fc[j];
+          if (on) K[j] = 0;
           any_live = any_live | live[j];
         }
         cnt.w.uniforms += (unsigned int)__builtin_popcount(inv);
@@ -556,78 +552,19 @@ struct EngineF32 {
         for (int j = 0; j < MP; ++j)
           if ((inv >> j) & 1u) cnt.w.inv += (unsigned int)K[j];
       }
-      // means of 2^22 and above: FP64 PTRS, one reaction at a time (rare)
-      unsigned bigp = 0;
-#pragma unroll
-      for (int j = 0; j < MP; ++j) bigp |= (((pend >> j) & 1u) && lam[j] >= BIG_LAMBDA_F32) ? (1u << j) : 0u;
-      pend &= ~bigp;
-      if (bigp != 0u) {
+      // means of 2^16 and above (rare): PTRS, one reaction at a time
+      if (wave_any(wide != 0u)) {
 #pragma unroll
         for (int j = 0; j < MP; ++j) {
-          if ((bigp >> j) & 1u) {
-            const Draw dr = ptrs_big_f64(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 1), j & 1,
-                                         (double)lam[j]);
+          if ((wide >> j) & 1u) {
+            const Draw dr = ptrs_wide(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)(j >> 2), j & 3,
+                                      lam[j]);
             cnt.w.ptrs += dr.iters;
             cnt.w.uniforms += 2 * dr.iters;
             if (dr.k < 0) return FLAG_PTRS;
             K[j] = (int32_t)dr.k;
           }
         }
-      }
-      // FP32 draws rejected at iteration 0: iteration `it` of reaction j uses
-      // half j % 2 of block (grp + j / 2, it).  Each lane works through its
-      // pending reactions in index order and moves to the next one as soon as
-      // a draw is accepted, so the wavefront runs the largest per-lane total
-      // of retries instead of one retry loop per pending reaction.
-      // The lane reads the mean of its current reaction from entry 0 of a
-      // register queue of its pending means (in index order), shifted when a
-      // draw is accepted: with a run-time index into lam the compiler would
-      // place lam in LDS and every read would wait for it.
-      constexpr int QN = MP <= 8 ? MP : 1;
-      float ql[QN];
-      if constexpr (MP <= 8) {
-        int nq = 0;
-#pragma unroll
-        for (int q = 0; q < QN; ++q) ql[q] = 0.0f;
-#pragma unroll
-        for (int j = 0; j < MP; ++j) {
-          const bool on = (pend >> j) & 1u;
-#pragma unroll
-          for (int q = 0; q <= j; ++q) ql[q] = (on && nq == q) ? lam[j] : ql[q];
-          nq += on ? 1 : 0;
-        }
-      }
-      {
-        uint32_t it = 1;
-        bool fail = false;
-        while (pend != 0u) {
-          const int j = __builtin_ctz(pend);
-          float lj;
-          if constexpr (MP <= 8) {
-            lj = ql[0];
-          } else {
-            lj = pick_m<MP>(lam, j);
-          }
-          const PtrsF32 Pt = ptrs_setup_f32(lj);
-          const Words4 bw = key.block(grp + (uint64_t)(j >> 1), it);
-          const float U = u24((j & 1) ? bw.w2 : bw.w0) - 0.5f;
-          const float V = u24((j & 1) ? bw.w3 : bw.w1);
-          cnt.w.ptrs += 1;
-          cnt.w.uniforms += 2;
-          float kf, us;
-          if (ptrs_fast_f32(Pt, U, V, kf, us) || ptrs_slow_f32(Pt, V, kf, us)) {
-#pragma unroll
-            for (int r = 0; r < MP; ++r) K[r] = (r == j) ? (int32_t)kf : K[r];
-            pend &= pend - 1u;
-            it = 1;
-#pragma unroll
-            for (int q = 0; q + 1 < QN; ++q) ql[q] = ql[q + 1];
-          } else if (++it >= (uint32_t)PTRS_MAX_ITERS) {
-            fail = true;
-            pend = 0u;
-          }
-        }
-        if (fail) return FLAG_PTRS;
       }
       bool ok = true, big = false;
 #pragma unroll

@@ -385,7 +385,7 @@ struct EngineWave {
         bad = bad || (l >= (float)LAMBDA_LIMIT);
         if (l > 0.0f && !bad) {
           const Words4 bw = key.block(grp + (uint64_t)j, 0);
-          if (l < 10.0f) {
+          if (l < POIS_ASYM_MIN) {
             // sequential-search inversion
             cnt.w.uniforms += 1;
             const float u = u24(bw.w0);
@@ -400,23 +400,20 @@ struct EngineWave {
             }
             cnt.w.inv += (unsigned int)n;
             kk = n;
-          } else if (l < BIG_LAMBDA_F32) {
-            cnt.w.ptrs += 1;
-            cnt.w.uniforms += 2;
-            const PtrsF32 Pt = ptrs_setup_f32(l);
-            const float V = u24(bw.w1);  // words of half 0, as in ptrs_rest_f32
-            float kf, us;
-            if (ptrs_fast_f32(Pt, u24(bw.w0) - 0.5f, V, kf, us) || ptrs_slow_f32(Pt, V, kf, us)) {
-              kk = (int32_t)kf;
-            } else {
-              const Draw d = ptrs_rest_f32(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)j, 0, 1, l);
-              cnt.w.ptrs += d.iters;
-              cnt.w.uniforms += 2 * d.iters;
-              if (d.k < 0) fail = true;
-              else kk = (int32_t)d.k;
+          } else if (l < POIS_ASYM_MAX) {
+            // asymptotic inversion, decided in FP64 near an integer boundary
+            cnt.w.asym += 1;
+            cnt.w.uniforms += 1;
+            const float u = u24(bw.w0);
+            bool ex = false;
+            int32_t m = -1;
+            kk = poisson_asym_f32(l, u, ex, m);
+            if (ex) {
+              cnt.w.exact += 1;
+              kk = poisson_exact_f64((double)l, (double)u, m);
             }
           } else {
-            const Draw d = ptrs_big_f64(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)j, 0, (double)l);
+            const Draw d = ptrs_wide(key.k0, key.k1, key.s_lo, key.s_hi, grp + (uint64_t)j, 0, l);
             cnt.w.ptrs += d.iters;
             cnt.w.uniforms += 2 * d.iters;
             if (d.k < 0) fail = true;

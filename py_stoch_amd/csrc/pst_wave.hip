@@ -41,11 +41,11 @@ sim_wave_kernel(const WaveJob* __restrict__ jobs, int n_jobs, unsigned int* __re
       }
     }
   }
-  const unsigned long long vals[ST_FEVENTS + 1] = {cnt.events, cnt.leaps, cnt.rejected, cnt.ssa, cnt.flagged,
-                                                   cnt.w.uniforms, cnt.w.ptrs, cnt.w.inv, cnt.w.decisions,
-                                                   cnt.w.samples, cnt.fevents};
+  const unsigned long long vals[ST_EXACT + 1] = {cnt.events, cnt.leaps, cnt.rejected, cnt.ssa, cnt.flagged,
+                                                 cnt.w.uniforms, cnt.w.ptrs, cnt.w.inv, cnt.w.decisions,
+                                                 cnt.w.samples, cnt.fevents, cnt.w.asym, cnt.w.exact};
 #pragma unroll
-  for (int i = 0; i <= ST_FEVENTS; ++i) {
+  for (int i = 0; i <= ST_EXACT; ++i) {
     const unsigned long long sum = wave_sum(vals[i]);
     if (lane == 0) atomicAdd(&stats[i], sum);
   }

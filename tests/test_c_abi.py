@@ -30,7 +30,7 @@ def test_library_exports_every_declared_function(native):
 
 
 def test_abi_version_and_limits(native):
-    assert native.load().pst_abi_version() == 4 == native.ABI_VERSION
+    assert native.load().pst_abi_version() == 5 == native.ABI_VERSION
     assert native.max_network() == (64, 256)
 
 
