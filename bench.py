@@ -227,6 +227,14 @@ def time_plan(plan, O, K, W, stream):
     return (t1 - t0), e0.elapsed_time(e1) / K, st
 
 
+def parse_launch(text):
+    """--launch SHIFT,SLICE[,TRIM[,BURST_WAVES]] -> the four launch-shape values."""
+    v = [int(x) for x in text.split(",")]
+    if not 2 <= len(v) <= 4:
+        raise SystemExit("--launch takes SHIFT,SLICE[,TRIM[,BURST_WAVES]]")
+    return tuple((v + [0, 0])[:4])
+
+
 def valid_events(st):
     """Reaction events of the simulations that ended unflagged."""
     return st["events"] - st["flagged_events"]
@@ -337,9 +345,9 @@ def other_workload(args):
     p_f64 = _native.BatchPlan(workloads.jobs(specs, _native.PRECISION_FP64))
     tune = None
     if args.launch:
-        shift, slc, trim = ([int(v) for v in args.launch.split(",")] + [0])[:3]
-        p_mix.set_launch(shift, slc, trim)
-        tune = {"mixed": {"lanes_shift": shift, "ssa_slice": slc, "waves_trim": trim, "set": True}}
+        shift, slc, trim, bw = parse_launch(args.launch)
+        p_mix.set_launch(shift, slc, trim, bw)
+        tune = {"mixed": {"lanes_shift": shift, "ssa_slice": slc, "waves_trim": trim, "burst_waves": bw, "set": True}}
     elif not args.no_tune:
         print("bench: %s, autotuning launch shapes" % wl, file=sys.stderr, flush=True)
         ptrs, strides = [O.data_ptr() for O in O_list], [s.n_sims for s in specs]
@@ -482,9 +490,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the library's default launch shapes (skip the launch autotuner)")
-    ap.add_argument("--launch", default=None, metavar="SHIFT,SLICE[,TRIM]",
-                    help="launch shape of the mixed plan (lanes_shift,ssa_slice,waves_trim of an autotune); implies "
-                         "--no-tune for it (to profile the shape a default run chose)")
+    ap.add_argument("--launch", default=None, metavar="SHIFT,SLICE[,TRIM[,BURST_WAVES]]",
+                    help="launch shape of the mixed plan (lanes_shift,ssa_slice,waves_trim,burst_waves of an "
+                         "autotune); implies --no-tune for it (to profile the shape a default run chose)")
     ap.add_argument("--acc-sets", type=int, default=16,
                     help="independent sets of 65536 simulations per path for the accuracy check")
     ap.add_argument("--workload", default="lotka-volterra",
@@ -537,7 +545,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         if args.launch:
-            p_mixed.set_launch(*(int(v) for v in args.launch.split(",")))
+            p_mixed.set_launch(*parse_launch(args.launch))
         p_mixed.execute(O_m.data_ptr(), U, None, stream.cuda_stream)
         torch.cuda.synchronize()
         print("bench: traffic probe done (copy %d bytes, one mixed launch)" % COPY_BYTES, file=sys.stderr)
@@ -547,10 +555,9 @@ def main():
     # wavefronts per SIMD) of every plan, timed on the whole horizon; results do not depend on them
     tune = None
     if args.launch:
-        shape = [int(v) for v in args.launch.split(",")] + [0]
-        shift, slc, trim = shape[:3]
-        p_mixed.set_launch(shift, slc, trim)
-        tune = {"mixed": {"lanes_shift": shift, "ssa_slice": slc, "waves_trim": trim, "set": True}}
+        shift, slc, trim, bw = parse_launch(args.launch)
+        p_mixed.set_launch(shift, slc, trim, bw)
+        tune = {"mixed": {"lanes_shift": shift, "ssa_slice": slc, "waves_trim": trim, "burst_waves": bw, "set": True}}
     elif not args.no_tune:
         print("bench: autotuning launch shapes", file=sys.stderr, flush=True)
         tune = {name: plan.autotune(O.data_ptr(), U, None, stream.cuda_stream, fraction=TUNE_FRACTION)

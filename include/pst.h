@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define PST_ABI_VERSION 5
+#define PST_ABI_VERSION 6
 
 /* status codes */
 #define PST_OK 0
@@ -214,7 +214,8 @@ int pst_batch_plan_execute(pst_plan* plan, int32_t* const* O_dev, const int64_t*
  * Launch autotuner.  Times candidate launch shapes of a plan -- simulations
  * per wavefront (the library's choice, and half of it) x SSA-only steps per
  * pass while lanes sit in a burst (3, 7, 15) x resident wavefronts per SIMD
- * (the kernel's occupancy limit, and one fewer) -- on the first `fraction` of
+ * (the kernel's occupancy limit, and one fewer) x SSA bursts of tau-leaping
+ * network kernels on paired wavefronts (off, on) -- on the first `fraction` of
  * every job's sample times (0 < fraction <= 1; writes into the given
  * buffers, which need the shapes of pst_batch_plan_execute), and keeps the
  * fastest for the plan's later executions (another shape replaces the
@@ -233,16 +234,20 @@ typedef struct pst_launch {
                                  than simulations, the work queue refills lanes and a
                                  wavefront no longer lasts as long as its slowest
                                  simulation */
-  int32_t reserved;
+  int32_t burst_waves;        /* 1: tau-leaping network kernels run in workgroups of four
+                                 decide and four burst wavefronts; a simulation's SSA
+                                 fallback burst (P3.py:71-151) moves through LDS to the
+                                 paired burst wavefront and back */
   double kernel_ms;           /* kernel time of the chosen shape on the tuning horizon */
   double default_ms;          /* same for the library's default shape (0, 7) */
 } pst_launch;
 int pst_plan_autotune(pst_plan* plan, int32_t* const* O_dev, const int64_t* o_strides,
                       int32_t* const* flags_dev, void* stream, double fraction, pst_launch* chosen);
 /* Set a launch shape directly (a pst_launch of an earlier autotune):
- * 0 <= lanes_shift <= 6, 0 <= ssa_slice <= 63, 0 <= waves_trim <= 7.  No
- * reference counterpart. */
-int pst_plan_set_launch(pst_plan* plan, int32_t lanes_shift, int32_t ssa_slice, int32_t waves_trim);
+ * 0 <= lanes_shift <= 6, 0 <= ssa_slice <= 63, 0 <= waves_trim <= 7,
+ * burst_waves 0 or 1.  No reference counterpart. */
+int pst_plan_set_launch(pst_plan* plan, int32_t lanes_shift, int32_t ssa_slice, int32_t waves_trim,
+                        int32_t burst_waves);
 
 /*
  * No counterpart in the reference: for callers that cannot allocate device

@@ -12,7 +12,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libpst_hip.so")
-ABI_VERSION = 5  # PST_ABI_VERSION of include/pst.h
+ABI_VERSION = 6  # PST_ABI_VERSION of include/pst.h
 
 METHOD_TAU_LEAPING = 0
 METHOD_FORCED_SSA = 1
@@ -102,14 +102,15 @@ class PstLaunch(ctypes.Structure):
         ("lanes_shift", ctypes.c_int32),
         ("ssa_slice", ctypes.c_int32),
         ("waves_trim", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("burst_waves", ctypes.c_int32),
         ("kernel_ms", ctypes.c_double),
         ("default_ms", ctypes.c_double),
     ]
 
     def as_dict(self):
         return {"lanes_shift": int(self.lanes_shift), "ssa_slice": int(self.ssa_slice),
-                "waves_trim": int(self.waves_trim), "kernel_ms": float(self.kernel_ms), "default_ms": float(self.default_ms)}
+                "waves_trim": int(self.waves_trim), "burst_waves": int(self.burst_waves),
+                "kernel_ms": float(self.kernel_ms), "default_ms": float(self.default_ms)}
 
 
 class PstJob(ctypes.Structure):
@@ -178,7 +179,8 @@ def load():
                                     ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_void_p),
                                     ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(PstLaunch)]
     L.pst_plan_autotune.restype = ctypes.c_int
-    L.pst_plan_set_launch.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+    L.pst_plan_set_launch.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_int32]
     L.pst_plan_set_launch.restype = ctypes.c_int
     L.pst_plan_create.argtypes = [ctypes.POINTER(PstNetwork), ctypes.POINTER(PstConfig), _P_F64,
                                   _P_I32, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
@@ -368,9 +370,10 @@ class Plan:
         """Re-key the random stream of the following executions."""
         _check(load().pst_plan_set_stream(self._h, int(seed), int(sim_offset)), "pst_plan_set_stream")
 
-    def set_launch(self, lanes_shift, ssa_slice, waves_trim=0):
+    def set_launch(self, lanes_shift, ssa_slice, waves_trim=0, burst_waves=0):
         """Use this launch shape (e.g. one an autotune returned) for later executions."""
-        _check(load().pst_plan_set_launch(self._h, int(lanes_shift), int(ssa_slice), int(waves_trim)),
+        _check(load().pst_plan_set_launch(self._h, int(lanes_shift), int(ssa_slice), int(waves_trim),
+                                          int(burst_waves)),
                "pst_plan_set_launch")
 
     def autotune(self, O_dev_ptr, o_stride, flags_dev_ptr=None, stream_ptr=None, fraction=0.25):
@@ -455,9 +458,10 @@ class BatchPlan:
                                              ctypes.c_void_p(int(stream_ptr)) if stream_ptr else None),
                "pst_batch_plan_execute")
 
-    def set_launch(self, lanes_shift, ssa_slice, waves_trim=0):
+    def set_launch(self, lanes_shift, ssa_slice, waves_trim=0, burst_waves=0):
         """Use this launch shape (e.g. one an autotune returned) for later executions."""
-        _check(load().pst_plan_set_launch(self._h, int(lanes_shift), int(ssa_slice), int(waves_trim)),
+        _check(load().pst_plan_set_launch(self._h, int(lanes_shift), int(ssa_slice), int(waves_trim),
+                                          int(burst_waves)),
                "pst_plan_set_launch")
 
     def autotune(self, O_ptrs, strides, flag_ptrs=None, stream_ptr=None, fraction=0.25):

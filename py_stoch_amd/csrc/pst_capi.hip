@@ -351,6 +351,10 @@ struct FixedGroup {
   pst::RtcKernel k_occ;     // compiled on first need
   bool occ_tried = false;
   bool use_occ = false;     // this execute launches k_occ
+  std::string pair_source;  // source of the paired build (decide + burst wavefronts)
+  pst::RtcKernel k_pair;    // compiled on first need
+  bool pair_tried = false;
+  bool use_pair = false;    // this execute launches k_pair
   std::vector<int> jobs;
   void* d_jobs = nullptr;
   size_t d_bytes = 0;
@@ -377,6 +381,7 @@ struct pst_plan {
   int lanes_shift = 0;        // launch shape (pst_plan_autotune)
   int ssa_slice = kSsaSlice;
   int waves_trim = 0;
+  int burst_waves = 0;        // 1: tau-leaping network kernels run bursts on paired wavefronts
   // extra streams for plans with several kernel launches (created on first need)
   hipStream_t side[kSideStreams] = {};
   hipEvent_t fork = nullptr;
@@ -426,7 +431,7 @@ static int enqueue_launches(pst_plan* p, const int* count, int n_wave, int64_t w
   for (FixedGroup& g : p->fixed) {
     int64_t need = 0;
     for (int q : g.jobs) need += ((int64_t)p->jobs[q].p.n_sims + p->jobs[q].p.lanes - 1) / p->jobs[q].p.lanes;
-    const pst::RtcKernel* k = g.use_occ ? &g.k_occ : &g.k;
+    const pst::RtcKernel* k = g.use_pair ? &g.k_pair : g.use_occ ? &g.k_occ : &g.k;
     const int64_t resident = (int64_t)trimmed(k->blocks_per_cu, p->waves_trim) * p->cu_count * 4;
     const int waves = (int)(need < resident ? need : resident);
     const int cnt = (int)g.jobs.size();
@@ -644,6 +649,7 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
         p->fixed.emplace_back();
         p->fixed[g].k = k;
         p->fixed[g].occ_source = pst::rtc_source(pj.net, P, p->precision, pj.hazards, kOccupancyWaves);
+        p->fixed[g].pair_source = pst::rtc_source(pj.net, P, p->precision, pj.hazards, 0, true);
       }
       p->fixed[g].jobs.push_back(q);
       pj.fixed = g;
@@ -723,6 +729,19 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
       if (rc != PST_OK) return fail(rc, "network kernel (occupancy build): %s", err.c_str());
     }
     g.use_occ = g.k_occ.fn != nullptr && g.k_occ.blocks_per_cu > g.k.blocks_per_cu;
+  }
+  // network kernels of tau-leaping jobs: SSA bursts on paired wavefronts
+  for (FixedGroup& g : p->fixed) {
+    g.use_pair = false;
+    if (!p->burst_waves || p->jobs[g.jobs[0]].p.method != PST_METHOD_TAU_LEAPING) continue;
+    if (!g.pair_tried) {
+      g.pair_tried = true;
+      std::string err;
+      const int rc = pst::rtc_kernel(g.pair_source, p->precision, &g.k_pair, &err, true);
+      if (rc != PST_OK) return fail(rc, "network kernel (paired build): %s", err.c_str());
+    }
+    if (!g.k_pair.fn) return fail(PST_EHIP, "network kernel (paired build) is not available");
+    g.use_pair = true;
   }
   // job descriptors, one device array per size class
   int count[pst::NUM_CLASSES] = {0, 0, 0};
@@ -893,13 +912,15 @@ int pst_plan_autotune(pst_plan* p, int32_t* const* O_dev, const int64_t* o_strid
     p->jobs[q].p.ita = std::max(1, std::min(ita[q], (int32_t)std::ceil(fraction * (double)ita[q])));
   }
   const int old_shift = p->lanes_shift, old_slice = p->ssa_slice, old_trim = p->waves_trim;
+  const int old_bw = p->burst_waves;
   auto restore = [&]() {
     for (size_t q = 0; q < p->jobs.size(); ++q) p->jobs[q].p.ita = ita[q];
   };
-  auto timed = [&](int shift, int slice, int trim, double* ms) -> int {
+  auto timed = [&](int shift, int slice, int trim, int bw, double* ms) -> int {
     p->lanes_shift = shift;
     p->ssa_slice = slice;
     p->waves_trim = trim;
+    p->burst_waves = bw;
     const int rc = pst_batch_plan_execute(p, O_dev, o_strides, flags_dev, stream);
     if (rc != PST_OK) return rc;
     hipError_t e = hipEventSynchronize(p->ev1);
@@ -910,22 +931,36 @@ int pst_plan_autotune(pst_plan* p, int32_t* const* O_dev, const int64_t* o_strid
     return PST_OK;
   };
   double ms = 0.0, def_ms = 0.0, best_ms = 0.0;
-  int best_shift = 0, best_slice = kSsaSlice, best_trim = 0;
-  int rc = timed(0, kSsaSlice, 0, &ms);  // warm-up (first launch of each kernel)
-  if (rc == PST_OK) rc = timed(0, kSsaSlice, 0, &def_ms);
+  int best_shift = 0, best_slice = kSsaSlice, best_trim = 0, best_bw = 0;
+  int rc = timed(0, kSsaSlice, 0, 0, &ms);  // warm-up (first launch of each kernel)
+  if (rc == PST_OK) rc = timed(0, kSsaSlice, 0, 0, &def_ms);
   best_ms = def_ms;
+  // paired wavefronts apply to network kernels of tau-leaping jobs; the SSA
+  // slice then matters only to the plan's other launches
+  bool pairable = false, others = false;
+  for (const FixedGroup& g : p->fixed) pairable = pairable || p->jobs[g.jobs[0]].p.method == PST_METHOD_TAU_LEAPING;
+  for (const PlanJob& pj : p->jobs) others = others || pj.fixed < 0;
   const int slices[3] = {3, 7, 15};
-  for (int trim = 0; trim <= 1 && rc == PST_OK; ++trim) {
-    for (int shift = 0; shift <= 1 && rc == PST_OK; ++shift) {
-      for (int k = 0; k < 3 && rc == PST_OK; ++k) {
-        if (trim == 0 && shift == 0 && slices[k] == kSsaSlice) continue;  // the default, timed above
-        rc = timed(shift, slices[k], trim, &ms);
-        // another shape must beat the default by 2 % (run-to-run noise)
-        if (rc == PST_OK && ms < best_ms && ms < 0.98 * def_ms) {
-          best_ms = ms;
-          best_shift = shift;
-          best_slice = slices[k];
-          best_trim = trim;
+  for (int bw = 0; bw <= (pairable ? 1 : 0) && rc == PST_OK; ++bw) {
+    for (int trim = 0; trim <= 1 && rc == PST_OK; ++trim) {
+      for (int shift = 0; shift <= 1 && rc == PST_OK; ++shift) {
+        for (int k = 0; k < 3 && rc == PST_OK; ++k) {
+          if (bw == 0 && trim == 0 && shift == 0 && slices[k] == kSsaSlice) continue;  // the default, timed above
+          if (bw == 1 && !others && slices[k] != kSsaSlice) continue;
+          rc = timed(shift, slices[k], trim, bw, &ms);
+          if (rc != PST_OK && bw == 1) {  // no paired build for this network (e.g. its state does not fit LDS)
+            rc = PST_OK;
+            g_err[0] = 0;
+            break;
+          }
+          // another shape must beat the default by 2 % (run-to-run noise)
+          if (rc == PST_OK && ms < best_ms && ms < 0.98 * def_ms) {
+            best_ms = ms;
+            best_shift = shift;
+            best_slice = slices[k];
+            best_trim = trim;
+            best_bw = bw;
+          }
         }
       }
     }
@@ -935,29 +970,34 @@ int pst_plan_autotune(pst_plan* p, int32_t* const* O_dev, const int64_t* o_strid
     p->lanes_shift = old_shift;
     p->ssa_slice = old_slice;
     p->waves_trim = old_trim;
+    p->burst_waves = old_bw;
     return rc;
   }
   p->lanes_shift = best_shift;
   p->ssa_slice = best_slice;
   p->waves_trim = best_trim;
+  p->burst_waves = best_bw;
   chosen->lanes_shift = best_shift;
   chosen->ssa_slice = best_slice;
   chosen->waves_trim = best_trim;
-  chosen->reserved = 0;
+  chosen->burst_waves = best_bw;
   chosen->kernel_ms = best_ms;
   chosen->default_ms = def_ms;
   return PST_OK;
 }
 
-int pst_plan_set_launch(pst_plan* p, int32_t lanes_shift, int32_t ssa_slice, int32_t waves_trim) {
+int pst_plan_set_launch(pst_plan* p, int32_t lanes_shift, int32_t ssa_slice, int32_t waves_trim,
+                        int32_t burst_waves) {
   g_err[0] = 0;
   if (!p) return fail(PST_EINVAL, "null plan");
   if (lanes_shift < 0 || lanes_shift > 6) return fail(PST_EINVAL, "lanes_shift %d outside [0, 6]", lanes_shift);
   if (ssa_slice < 0 || ssa_slice > 63) return fail(PST_EINVAL, "ssa_slice %d outside [0, 63]", ssa_slice);
   if (waves_trim < 0 || waves_trim > 7) return fail(PST_EINVAL, "waves_trim %d outside [0, 7]", waves_trim);
+  if (burst_waves < 0 || burst_waves > 1) return fail(PST_EINVAL, "burst_waves %d outside [0, 1]", burst_waves);
   p->lanes_shift = lanes_shift;
   p->ssa_slice = ssa_slice;
   p->waves_trim = waves_trim;
+  p->burst_waves = burst_waves;
   return PST_OK;
 }
 

@@ -168,4 +168,214 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
   }
 }
 
+// ---------------------------------------------------------------------------
+// Paired driver: a workgroup of 512 lanes is four decide wavefronts (0-3) and
+// four burst wavefronts (4-7), two wavefronts per SIMD.  A decide wavefront
+// runs its simulations like run_driver but takes no SSA-only steps: when a
+// leap decision switches a simulation into an SSA fallback burst
+// (P1_P2.py:157-173, then P3.py:71-151), the lane writes the simulation's
+// state to an LDS slot and lane l of burst wavefront w + 4 takes it, runs the
+// burst's remaining SSA steps back to back and hands the state back.  So the
+// leap passes of a decide wavefront no longer wait for other lanes' SSA steps,
+// a burst no longer waits for leap passes, and each SIMD overlaps the two
+// instruction streams (one wavefront per SIMD leaves half its issue slots to
+// latency).  Every simulation still executes exactly the reference's
+// sequence; only where its steps run changes.
+//
+// Slot protocol (one slot per decide lane): FREE -> SENT (decide lane, after
+// writing the state) -> BACK (burst lane, after writing the state back) ->
+// FREE (decide lane, after reading it).  A decide lane with a SENT slot is
+// AWAY and skips passes; a decide wavefront moves to its next job only when
+// none of its lanes is away, so every simulation a burst wavefront holds
+// belongs to its partner's current job.  A burst wavefront never waits for
+// its partner except while it has nothing to run, and it leaves once the
+// partner has finished every job (DONE flag), when no slot can be SENT.
+// ---------------------------------------------------------------------------
+enum { MODE_AWAY = 3 };
+enum { SLOT_FREE = 0u, SLOT_SENT = 1u, SLOT_BACK = 2u };
+constexpr int PAIR_SLOTS = 256;  // decide lanes per workgroup
+
+__device__ __forceinline__ uint32_t lds_load_acquire(uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_store_release(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// A simulation in transit: the engine state, its id, its events so far
+// (for the flagged-events counter), the job index and the next mode.
+template <class Engine>
+struct SlotIO {
+  static constexpr int EW = (int)(sizeof(Engine) / 4);
+  static constexpr int WORDS = EW + 6;
+  static_assert(sizeof(Engine) % 4 == 0, "engine state is not whole words");
+  // word-major [word][slot]: consecutive lanes hit consecutive banks
+  static __device__ __forceinline__ void put(uint32_t* data, int slot, const Engine& eng, int64_t sim,
+                                             unsigned long long ev, int job, int mode) {
+    uint32_t w[EW];
+    __builtin_memcpy(w, &eng, sizeof(Engine));
+#pragma unroll
+    for (int i = 0; i < EW; ++i) data[i * PAIR_SLOTS + slot] = w[i];
+    data[(EW + 0) * PAIR_SLOTS + slot] = (uint32_t)sim;
+    data[(EW + 1) * PAIR_SLOTS + slot] = (uint32_t)((uint64_t)sim >> 32);
+    data[(EW + 2) * PAIR_SLOTS + slot] = (uint32_t)ev;
+    data[(EW + 3) * PAIR_SLOTS + slot] = (uint32_t)(ev >> 32);
+    data[(EW + 4) * PAIR_SLOTS + slot] = (uint32_t)job;
+    data[(EW + 5) * PAIR_SLOTS + slot] = (uint32_t)mode;
+  }
+  static __device__ __forceinline__ void get(const uint32_t* data, int slot, Engine& eng, int64_t& sim,
+                                             unsigned long long& ev, int& mode) {
+    uint32_t w[EW];
+#pragma unroll
+    for (int i = 0; i < EW; ++i) w[i] = data[i * PAIR_SLOTS + slot];
+    __builtin_memcpy(&eng, w, sizeof(Engine));
+    sim = (int64_t)(((uint64_t)data[(EW + 1) * PAIR_SLOTS + slot] << 32) | data[(EW + 0) * PAIR_SLOTS + slot]);
+    ev = ((unsigned long long)data[(EW + 3) * PAIR_SLOTS + slot] << 32) | data[(EW + 2) * PAIR_SLOTS + slot];
+    mode = (int)data[(EW + 5) * PAIR_SLOTS + slot];
+  }
+  static __device__ __forceinline__ int job(const uint32_t* data, int slot) {
+    return (int)data[(EW + 4) * PAIR_SLOTS + slot];
+  }
+};
+
+template <class Engine, class Ref>
+__device__ __forceinline__ void finish_sim(const Ref& jr, const Engine& eng, int64_t sim, Counters& cnt,
+                                           unsigned long long ev_start) {
+  int32_t* fl = jr.p().flags;
+  if (fl) fl[sim] = eng.flag;
+  cnt.flagged += (eng.flag != 0);
+  cnt.fevents += (eng.flag != 0) ? cnt.events - ev_start : 0ull;
+}
+
+template <class Engine, class Ref>
+__device__ __forceinline__ void run_driver_paired(const typename Ref::Dev* __restrict__ jobs, int n_jobs,
+                                                  unsigned int* __restrict__ queue,
+                                                  unsigned long long* __restrict__ stats) {
+  using IO = SlotIO<Engine>;
+  __shared__ uint32_t s_data[IO::WORDS * PAIR_SLOTS];
+  __shared__ uint32_t s_state[PAIR_SLOTS];
+  __shared__ uint32_t s_done[4];
+  const int tid = (int)threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;  // wave-uniform
+  const int pair = wave & 3;
+  const int slot = pair * 64 + lane;
+  if (tid < PAIR_SLOTS) s_state[tid] = SLOT_FREE;
+  if (tid < 4) s_done[tid] = 0u;
+  __syncthreads();  // the only barrier: every lane of the workgroup reaches it
+
+  Counters cnt = {};
+  Engine eng;
+  int64_t sim = -1;
+  unsigned long long ev_start = 0;  // cnt.events when this lane's simulation began (here)
+  Ref jr;
+
+  if (wave < 4) {
+    // ---- decide wavefront ----
+    int mode = MODE_DONE;
+    int job = 0;
+    bool exhausted = false;
+    jr.load(jobs);
+    for (;;) {
+      // a burst handed back
+      if (mode == MODE_AWAY && lds_load_acquire(&s_state[slot]) == SLOT_BACK) {
+        unsigned long long ev;
+        IO::get(s_data, slot, eng, sim, ev, mode);
+        ev_start = cnt.events - ev;
+        s_state[slot] = SLOT_FREE;
+      }
+      const bool want = (mode == MODE_DONE) && !exhausted && lane < jr.p().lanes;
+      const unsigned long long wmask = __builtin_amdgcn_ballot_w64(want);
+      if (wmask != 0ull) {
+        const int n = __builtin_popcountll(wmask);
+        const int leader = __builtin_ctzll(wmask);
+        unsigned int base = 0;
+        if (lane == leader) base = atomicAdd(&queue[job], (unsigned int)n);
+        base = __shfl(base, leader, 64);
+        const int n_sims = jr.p().n_sims;
+        if ((int64_t)base + n >= (int64_t)n_sims) exhausted = true;
+        if (want) {
+          const int rank = __builtin_popcountll(wmask & ((1ull << lane) - 1ull));
+          const int64_t s = (int64_t)base + rank;
+          if (s < n_sims) {
+            sim = s;
+            ev_start = cnt.events;
+            mode = eng.start(jr, s);
+          }
+        }
+      }
+      if (!wave_any(mode != MODE_DONE)) {
+        job += 1;
+        if (job >= n_jobs) break;
+        jr.load(jobs + job);
+        exhausted = false;
+        continue;
+      }
+      if (wave_any(mode == MODE_DECIDE)) {
+        jr.refresh();
+        if (mode == MODE_DECIDE) {
+          mode = eng.template step<false>(jr, cnt, mode);
+          if (mode == MODE_DONE) finish_sim(jr, eng, sim, cnt, ev_start);
+        }
+      } else if (!wave_any(mode == MODE_SSA)) {
+        __builtin_amdgcn_s_sleep(2);  // every live lane is away: wait for a burst to come back
+      }
+      // a burst starts (or a forced-SSA simulation starts): hand it over
+      if (mode == MODE_SSA) {
+        IO::put(s_data, slot, eng, sim, cnt.events - ev_start, job, MODE_SSA);
+        lds_store_release(&s_state[slot], SLOT_SENT);
+        mode = MODE_AWAY;
+      }
+    }
+    lds_store_release(&s_done[pair], 1u);
+  } else {
+    // ---- burst wavefront ----
+    bool busy = false;
+    int bjob = -1;
+    for (;;) {
+      const bool pick = !busy && lds_load_acquire(&s_state[slot]) == SLOT_SENT;
+      const unsigned long long pmask = __builtin_amdgcn_ballot_w64(pick);
+      if (pmask != 0ull) {
+        // all simulations the partner sends belong to its current job
+        const int pj = __builtin_amdgcn_readlane(IO::job(s_data, slot), __builtin_ctzll(pmask));
+        if (pj != bjob) {
+          jr.load(jobs + pj);
+          bjob = pj;
+        }
+        if (pick) {
+          unsigned long long ev;
+          int m;
+          IO::get(s_data, slot, eng, sim, ev, m);
+          ev_start = cnt.events - ev;
+          busy = true;
+        }
+      }
+      if (!wave_any(busy)) {
+        if (lds_load_acquire(&s_done[pair]) != 0u) break;
+        __builtin_amdgcn_s_sleep(4);
+        continue;
+      }
+      jr.refresh();
+      if (busy) {
+        const int m = eng.template step<true>(jr, cnt, MODE_SSA);
+        if (m != MODE_SSA) {
+          if (m == MODE_DONE) finish_sim(jr, eng, sim, cnt, ev_start);
+          IO::put(s_data, slot, eng, sim, cnt.events - ev_start, bjob, m);
+          lds_store_release(&s_state[slot], SLOT_BACK);
+          busy = false;
+        }
+      }
+    }
+  }
+
+  const unsigned long long vals[ST_EXACT + 1] = {cnt.events, cnt.leaps, cnt.rejected, cnt.ssa, cnt.flagged,
+                                                 cnt.w.uniforms, cnt.w.ptrs, cnt.w.inv, cnt.w.decisions,
+                                                 cnt.w.samples, cnt.fevents, cnt.w.asym, cnt.w.exact};
+#pragma unroll
+  for (int i = 0; i <= ST_EXACT; ++i) {
+    const unsigned long long sum = wave_sum(vals[i]);
+    if (lane == 0) atomicAdd(&stats[i], sum);
+  }
+}
+
 }  // namespace pst

@@ -133,7 +133,7 @@ int compile(const std::string& source, int precision, std::vector<char>* code, s
 }  // namespace
 
 std::string rtc_source(const HostNet& net, const JobParams& c, int precision,
-                       const std::vector<std::string>& hazards, int waves_per_eu) {
+                       const std::vector<std::string>& hazards, int waves_per_eu, bool paired) {
   const int N = net.N, M = net.M;
   std::vector<int8_t> v(N * M), ord(N * M), stp(N * M), E(N, 0);
   for (int i = 0; i < N * M; ++i) {
@@ -170,19 +170,20 @@ std::string rtc_source(const HostNet& net, const JobParams& c, int precision,
     s += "#undef X\n#undef C\n";
   }
   s += "  }\n};\n}  // namespace pst\n";
-  s += precision == PST_PRECISION_FP64 ? "#define PST_ENGINE EngineF64\n#define PST_KERNEL_NAME pst_net_fp64\n"
-                                       : "#define PST_ENGINE EngineF32\n#define PST_KERNEL_NAME pst_net_mixed\n";
-  s += "extern \"C\" __global__ void __launch_bounds__(256)";
+  s += precision == PST_PRECISION_FP64 ? "#define PST_ENGINE EngineF64\n" : "#define PST_ENGINE EngineF32\n";
+  s += std::string("#define PST_KERNEL_NAME ") + (precision == PST_PRECISION_FP64 ? "pst_net_fp64" : "pst_net_mixed") +
+       (paired ? "_paired" : "") + "\n";
+  s += paired ? "extern \"C\" __global__ void __launch_bounds__(512)" : "extern \"C\" __global__ void __launch_bounds__(256)";
   if (waves_per_eu > 0) s += " __attribute__((amdgpu_waves_per_eu(" + std::to_string(waves_per_eu) + ")))";
   s += "\n"
        "PST_KERNEL_NAME(const pst::JobParams* __restrict__ jobs, int n_jobs, unsigned int* __restrict__ queue,\n"
-       "                 unsigned long long* __restrict__ stats) {\n"
-       "  pst::run_driver<pst::PST_ENGINE<pst::FixedRef<pst::NetGen>>, pst::FixedRef<pst::NetGen>>(jobs, n_jobs, "
-       "queue, stats);\n}\n";
+       "                 unsigned long long* __restrict__ stats) {\n";
+  s += std::string("  pst::") + (paired ? "run_driver_paired" : "run_driver") +
+       "<pst::PST_ENGINE<pst::FixedRef<pst::NetGen>>, pst::FixedRef<pst::NetGen>>(jobs, n_jobs, queue, stats);\n}\n";
   return s;
 }
 
-int rtc_kernel(const std::string& source, int precision, RtcKernel* out, std::string* err) {
+int rtc_kernel(const std::string& source, int precision, RtcKernel* out, std::string* err, bool paired) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) {
@@ -230,13 +231,15 @@ int rtc_kernel(const std::string& source, int precision, RtcKernel* out, std::st
     return PST_EHIP;
   }
   RtcKernel k;
-  if ((e = hipModuleGetFunction(&k.fn, mod, precision == PST_PRECISION_FP64 ? "pst_net_fp64" : "pst_net_mixed")) !=
-      hipSuccess) {
+  const std::string name = std::string(precision == PST_PRECISION_FP64 ? "pst_net_fp64" : "pst_net_mixed") +
+                           (paired ? "_paired" : "");
+  if ((e = hipModuleGetFunction(&k.fn, mod, name.c_str())) != hipSuccess) {
     *err = std::string("hipModuleGetFunction: ") + hipGetErrorString(e);
     return PST_EHIP;
   }
+  k.block = paired ? 512 : 256;
   int blocks = 0;
-  if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k.fn, 256, 0) == hipSuccess && blocks > 0)
+  if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k.fn, k.block, 0) == hipSuccess && blocks > 0)
     k.blocks_per_cu = blocks;
   g_cache.emplace(key, k);  // the module stays loaded for the life of the process
   *out = k;
@@ -245,10 +248,10 @@ int rtc_kernel(const std::string& source, int precision, RtcKernel* out, std::st
 
 hipError_t rtc_launch(const RtcKernel& k, const JobParams* jobs, int n_jobs, unsigned int* queue,
                       unsigned long long* stats, int waves, hipStream_t stream) {
-  const int grid = (waves + 3) / 4;
+  const int grid = (waves + 3) / 4;  // four simulation wavefronts per workgroup
   if (grid <= 0) return hipSuccess;
   void* args[] = {(void*)&jobs, (void*)&n_jobs, (void*)&queue, (void*)&stats};
-  return hipModuleLaunchKernel(k.fn, grid, 1, 1, 256, 1, 1, 0, stream, args, nullptr);
+  return hipModuleLaunchKernel(k.fn, grid, 1, 1, k.block, 1, 1, 0, stream, args, nullptr);
 }
 
 }  // namespace pst

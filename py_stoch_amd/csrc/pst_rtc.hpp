@@ -26,7 +26,8 @@ constexpr int RTC_MAX_REACTIONS = 32;
 
 struct RtcKernel {
   hipFunction_t fn = nullptr;
-  int blocks_per_cu = 1;  // resident 256-lane blocks per CU
+  int block = 256;        // lanes per workgroup: 256, or 512 for the paired driver
+  int blocks_per_cu = 1;  // resident workgroups per CU (each with four simulation wavefronts)
 };
 
 // Source text of the specialised kernel for a network and the constant
@@ -35,15 +36,18 @@ struct RtcKernel {
 // C expression per reaction (X(i), C(k), R(v), see pst_network.hazards).
 // waves_per_eu > 0 asks the compiler for register use that fits that many
 // wavefronts per SIMD (the kernel's arithmetic, and so its results, do not
-// change).
+// change).  paired: the kernel runs run_driver_paired (workgroups of 512
+// lanes: four decide and four burst wavefronts), named pst_net_*_paired.
 std::string rtc_source(const HostNet& net, const JobParams& c, int precision,
-                       const std::vector<std::string>& hazards, int waves_per_eu = 0);
+                       const std::vector<std::string>& hazards, int waves_per_eu = 0, bool paired = false);
 
 // Compile (or fetch from the cache) the kernel for `source` on the current
-// device.  Returns 0, or a PST_* status with a message in err.
-int rtc_kernel(const std::string& source, int precision, RtcKernel* out, std::string* err);
+// device (`paired` as given to rtc_source).  Returns 0, or a PST_* status
+// with a message in err.
+int rtc_kernel(const std::string& source, int precision, RtcKernel* out, std::string* err, bool paired = false);
 
-// Launch a specialised kernel: `jobs` is a device array of n_jobs JobParams.
+// Launch a specialised kernel: `jobs` is a device array of n_jobs JobParams;
+// `waves` simulation wavefronts, four per workgroup.
 hipError_t rtc_launch(const RtcKernel& k, const JobParams* jobs, int n_jobs, unsigned int* queue,
                       unsigned long long* stats, int waves, hipStream_t stream);
 

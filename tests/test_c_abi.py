@@ -30,7 +30,7 @@ def test_library_exports_every_declared_function(native):
 
 
 def test_abi_version_and_limits(native):
-    assert native.load().pst_abi_version() == 5 == native.ABI_VERSION
+    assert native.load().pst_abi_version() == 6 == native.ABI_VERSION
     assert native.max_network() == (64, 256)
 
 
@@ -106,7 +106,7 @@ def test_autotune_argument_checks(native):
 
 def test_set_launch_argument_checks(native):
     L = native.load()
-    assert L.pst_plan_set_launch(None, 0, 7, 0) == -1  # PST_EINVAL: null plan
+    assert L.pst_plan_set_launch(None, 0, 7, 0, 0) == -1  # PST_EINVAL: null plan
 
 
 def test_noalloc_argument_checks(native):

@@ -8,6 +8,8 @@ import pytest
 from py_stoch_amd import models
 from py_stoch_amd.mod.parser.tl_parser import tables_of
 
+from devbuf import DeviceArray as _DeviceArray
+
 pytestmark = pytest.mark.gpu
 
 
@@ -185,29 +187,6 @@ def test_immigration_birth_death_mean(native, precision):
     mean = O[0].mean(axis=-1)
     se = np.sqrt(np.maximum(O[0].var(axis=-1), 1.0) / O.shape[-1])
     assert np.all(np.abs(mean - m) <= 4 * se + 0.02 * m)
-
-
-class _DeviceArray:
-    """int32 device buffer from the HIP runtime the library links against
-    (libamdhip64.so.7, already loaded by it, so one runtime instance)."""
-
-    def __init__(self, shape):
-        import ctypes
-        self.hip = ctypes.CDLL("libamdhip64.so.7")
-        self.shape = tuple(shape)
-        self.nbytes = int(np.prod(self.shape)) * 4
-        self.ptr = ctypes.c_void_p()
-        assert self.hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(self.nbytes)) == 0
-
-    def get(self):
-        import ctypes
-        out = np.empty(self.shape, np.int32)
-        assert self.hip.hipDeviceSynchronize() == 0
-        assert self.hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, ctypes.c_size_t(self.nbytes), 2) == 0
-        return out
-
-    def free(self):
-        self.hip.hipFree(self.ptr)
 
 
 @pytest.mark.parametrize("precision", [0, 1])
