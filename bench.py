@@ -177,19 +177,24 @@ def ks_max(O_a, O_b):
     return float((fa - fb).abs().max().item())
 
 
-def accuracy(p_mixed, p_fp64, O, flags, stream, sets):
+def accuracy(p_mixed, p_fp64, O_blk, flags, stream, sets):
     """Mixed vs FP64 on `sets` independent sets of U simulations each, next to
     the same statistics between two independent FP64 sample sets (the noise
-    floor of a comparison of this size)."""
+    floor of a comparison of this size).  The plans write blocks into O_blk;
+    each run is converted to rows for the statistics."""
     import torch
+    from py_stoch_amd import _native
     cap = CAP_FACTOR * X0
     mix, ref, ref2 = Moments(cap), Moments(cap), Moments(cap)
+    kappa = O_blk.shape[0]
+    O = torch.empty((kappa, ITA, U), dtype=torch.int32, device=O_blk.device)
     O_first = {}
     for r in range(sets):
         for name, plan, acc, off in (("mixed", p_mixed, mix, r), ("fp64", p_fp64, ref, r),
                                      ("fp64b", p_fp64, ref2, sets + r)):
             plan.set_stream(SEED, off * U)
-            plan.execute(O.data_ptr(), U, flags.data_ptr(), stream.cuda_stream)
+            plan.execute(O_blk.data_ptr(), U, flags.data_ptr(), stream.cuda_stream)
+            _native.blocks_to_rows(O_blk.data_ptr(), U, O.data_ptr(), U, kappa, ITA, U, stream.cuda_stream)
             torch.cuda.synchronize()
             acc.add(O, flags)
             if r == 0:
@@ -525,8 +530,11 @@ def main():
     tabs = tables_of(a)
     rates = a.rate_fn(a.c)
     kappa = 2
+    # every plan writes the blocked trajectory layout (32-byte sectors of
+    # eight samples of one simulation, each written once); the accuracy
+    # check converts to rows
     common = dict(method=_native.METHOD_TAU_LEAPING, n_c=10, eta=0.03, ssa_burst=100, t_max=T_END,
-                  I=a.I, E=a.E, n_sims=U, seed=SEED)
+                  I=a.I, E=a.E, n_sims=U, seed=SEED, layout=_native.LAYOUT_BLOCKS)
     torch.cuda.init()
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
@@ -535,8 +543,9 @@ def main():
     p_mixed = _native.Plan(tabs, rates, a.x_0, precision=_native.PRECISION_MIXED, specialize=True, **common)
     p_fp64 = _native.Plan(tabs, rates, a.x_0, precision=_native.PRECISION_FP64, specialize=True, **common)
     p_plain = _native.Plan(tabs, rates, a.x_0, precision=_native.PRECISION_FP64, **common)
-    O_m = torch.empty((kappa, ITA, U), dtype=torch.int32, device=dev)
-    O_f = torch.empty((kappa, ITA, U), dtype=torch.int32, device=dev)
+    ita8 = (ITA + 7) // 8
+    O_m = torch.empty((kappa, ita8, U, 8), dtype=torch.int32, device=dev)
+    O_f = torch.empty((kappa, ita8, U, 8), dtype=torch.int32, device=dev)
     if args.traffic_probe:
         # reference copy of known size: FETCH_SIZE / WRITE_SIZE per byte moved
         src = torch.ones((COPY_BYTES // 4,), dtype=torch.int32, device=dev)
@@ -611,6 +620,7 @@ def main():
         "config": {"workload": "lotka-volterra tau-leaping, 65536 simulations, t_end=10, 1000 samples, "
                                "eps=0.03, n_c=10, SSA burst 100, mixed precision, network compiled into "
                                "a persistent batched kernel",
+                   "trajectory_layout": "blocks [kappa][ita/8][U][8] (every plan)",
                    "simulations": U, "t_end": T_END, "samples": ITA, "species_recorded": kappa,
                    "seed": SEED, "events_per_step": ev, "events_per_step_incl_flagged": st_m["events"],
                    "flagged_simulations": st_m["flagged"]},

@@ -61,6 +61,14 @@ extern "C" {
 #define PST_PRECISION_FP64 0      /* plain FP64 (bit-exact vs the CPU restatement) */
 #define PST_PRECISION_MIXED 1     /* FP32 propensities/tau, FP64 clock and selection */
 
+/* trajectory layout of a device array O (pst_config.layout) */
+#define PST_LAYOUT_ROWS 0         /* O[kappa][ita][o_stride]: the reference's [kappa][ita][U]
+                                     (cuTauLeaping.py:126)                          */
+#define PST_LAYOUT_BLOCKS 1       /* O[kappa][ceil(ita/8)][o_stride][8]: the 8 consecutive
+                                     samples of one simulation fill one 32-byte sector, so
+                                     each is written once, whole; the last block is padded
+                                     with zeros.  pst_blocks_to_rows converts.        */
+
 /* per-simulation flag bits */
 #define PST_FLAG_OVERFLOW 1   /* a count or a Poisson mean exceeded 2^30       */
 #define PST_FLAG_HALVING 2    /* tau halved 2000 times without a valid state   */
@@ -112,7 +120,9 @@ typedef struct pst_config {
                                  settings), as the reference compiles each
                                  model's kernels (cuTauLeaping.py:34-35);
                                  0: table-driven kernel of the library      */
-  int32_t reserved;           /* must be 0                                   */
+  int32_t layout;             /* PST_LAYOUT_* of the device array a plan or
+                                 pst_run_noalloc writes; the calls that
+                                 return host arrays take PST_LAYOUT_ROWS    */
   int64_t max_steps;          /* 0: none; else at most this many leap
                                  decisions + SSA steps per simulation, after
                                  which it ends with PST_FLAG_STEPS (bounds the
@@ -284,6 +294,16 @@ int pst_run_noalloc(const pst_network* net, const pst_config* cfg, const double*
  * with P(X <= n) >= u.  n <= 2^28.
  */
 int pst_poisson_probe_mixed(const float* lam, const uint32_t* words, int64_t n, int32_t* k, int32_t* path);
+
+/*
+ * No counterpart in the reference: convert device trajectories from
+ * PST_LAYOUT_BLOCKS (blocks[kappa][ceil(ita/8)][blocks_stride][8]) to
+ * PST_LAYOUT_ROWS (rows[kappa][ita][rows_stride]) for simulations
+ * 0..n_sims-1, on `stream` (returns without waiting).  The two arrays must
+ * not overlap.
+ */
+int pst_blocks_to_rows(const int32_t* blocks, int64_t blocks_stride, int32_t* rows, int64_t rows_stride,
+                       int32_t kappa, int32_t ita, int64_t n_sims, void* stream);
 
 /* largest network the compiled kernels accept (species, reactions) */
 void pst_max_network(int32_t* max_species, int32_t* max_reactions);

@@ -20,6 +20,8 @@ METHOD_PURE_SSA = 2
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+LAYOUT_ROWS = 0      # O[kappa][ita][o_stride]
+LAYOUT_BLOCKS = 1    # O[kappa][ceil(ita/8)][o_stride][8] (device arrays of plans and run_noalloc)
 
 FLAG_OVERFLOW = 1
 FLAG_HALVING = 2
@@ -68,7 +70,7 @@ class PstConfig(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("sim_offset", ctypes.c_uint64),
         ("specialize", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("layout", ctypes.c_int32),
         ("max_steps", ctypes.c_int64),
     ]
 
@@ -141,6 +143,7 @@ EXPORTED_SYMBOLS = (
     "pst_abi_version",
     "pst_plan_autotune",
     "pst_plan_set_launch",
+    "pst_blocks_to_rows",
 )
 
 _lib = None
@@ -182,6 +185,9 @@ def load():
     L.pst_plan_set_launch.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32]
     L.pst_plan_set_launch.restype = ctypes.c_int
+    L.pst_blocks_to_rows.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                     ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p]
+    L.pst_blocks_to_rows.restype = ctypes.c_int
     L.pst_plan_create.argtypes = [ctypes.POINTER(PstNetwork), ctypes.POINTER(PstConfig), _P_F64,
                                   _P_I32, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
     L.pst_plan_create.restype = ctypes.c_int
@@ -228,7 +234,7 @@ class _Marshalled:
     """Keeps the numpy arrays behind a PstNetwork/PstConfig alive."""
 
     def __init__(self, tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I, E,
-                 n_sims, sims_per_set, seed, sim_offset, specialize=False, max_steps=0):
+                 n_sims, sims_per_set, seed, sim_offset, specialize=False, max_steps=0, layout=0):
         N, M = int(tables["N"]), int(tables["M"])
         self.A = np.ascontiguousarray(tables["A"], np.int32).reshape(-1, 4)
         self.V = np.ascontiguousarray(tables["V"], np.int32).reshape(-1, 4)
@@ -263,7 +269,7 @@ class _Marshalled:
                              float(t_max), self.I.shape[0], self.E.shape[0],
                              _ptr(self.I, ctypes.c_float), _ptr(self.E, ctypes.c_int32),
                              int(n_sims), int(sims_per_set), int(seed), int(sim_offset),
-                             1 if specialize else 0, 0, int(max_steps))
+                             1 if specialize else 0, int(layout), int(max_steps))
 
 
 def simulate(tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64, n_c=10,
@@ -290,12 +296,13 @@ def simulate(tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_F
 def run_noalloc(tables, rates_dev_ptr, x0_dev_ptr, n_sets, I_dev_ptr, O_dev_ptr, o_stride, flags_dev_ptr=None,
                 stream_ptr=None, read_stats=True, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64, n_c=10,
                 eta=0.03, ssa_burst=100, t_max=1.0, ita=None, E=(0,), n_sims=1, sims_per_set=None, seed=0,
-                sim_offset=0, max_steps=0):
+                sim_offset=0, max_steps=0, layout=LAYOUT_ROWS):
     """pst_run_noalloc: one run on the table-driven kernel with every input
     and output in caller device memory (rates [n_sets][M] float64, x0
     [n_sets][N] int32, sample times [ita] float32, O [kappa][ita][o_stride]
-    int32, flags) and no device allocation by the library.  Returns the
-    counters (waiting for the run) if read_stats, else None (enqueued only)."""
+    int32 (or blocks, layout=LAYOUT_BLOCKS), flags) and no device allocation
+    by the library.  Returns the counters (waiting for the run) if
+    read_stats, else None (enqueued only)."""
     L = load()
     N, M = int(tables["N"]), int(tables["M"])
     if sims_per_set is None:
@@ -304,13 +311,23 @@ def run_noalloc(tables, rates_dev_ptr, x0_dev_ptr, n_sets, I_dev_ptr, O_dev_ptr,
         sims_per_set = max(1, int(n_sims))
     m = _Marshalled(tables, np.zeros((1, M)), np.zeros((1, N), np.int32), method, precision, n_c, eta, ssa_burst,
                     t_max, np.zeros(int(ita), np.float32), E, n_sims, sims_per_set, seed, sim_offset, False,
-                    max_steps)
+                    max_steps, layout)
     st = PstStats() if read_stats else None
     rc = L.pst_run_noalloc(ctypes.byref(m.net), ctypes.byref(m.cfg), rates_dev_ptr, x0_dev_ptr, int(n_sets),
                            I_dev_ptr, O_dev_ptr, int(o_stride), flags_dev_ptr, stream_ptr,
                            ctypes.byref(st) if st is not None else None)
     _check(rc, "pst_run_noalloc")
     return st.as_dict() if st is not None else None
+
+
+def blocks_to_rows(blocks_dev_ptr, blocks_stride, rows_dev_ptr, rows_stride, kappa, ita, n_sims, stream_ptr=None):
+    """pst_blocks_to_rows: device trajectories from LAYOUT_BLOCKS
+    [kappa][ceil(ita/8)][blocks_stride][8] to LAYOUT_ROWS
+    [kappa][ita][rows_stride] (enqueued on the stream)."""
+    _check(load().pst_blocks_to_rows(ctypes.c_void_p(int(blocks_dev_ptr)), int(blocks_stride),
+                                     ctypes.c_void_p(int(rows_dev_ptr)), int(rows_stride), int(kappa), int(ita),
+                                     int(n_sims), ctypes.c_void_p(int(stream_ptr)) if stream_ptr else None),
+           "pst_blocks_to_rows")
 
 
 def poisson_probe_mixed(lam, words):
@@ -345,10 +362,11 @@ class Plan:
 
     def __init__(self, tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64,
                  n_c=10, eta=0.03, ssa_burst=100, t_max=1.0, I=None, E=(0,), n_sims=1,
-                 sims_per_set=None, seed=0, sim_offset=0, specialize=False, max_steps=0):
+                 sims_per_set=None, seed=0, sim_offset=0, specialize=False, max_steps=0, layout=LAYOUT_ROWS):
         L = load()
         self._m = _Marshalled(tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I,
-                              E, n_sims, sims_per_set, seed, sim_offset, specialize, max_steps)
+                              E, n_sims, sims_per_set, seed, sim_offset, specialize, max_steps, layout)
+        self.layout = int(layout)
         self.n_sims = int(n_sims)
         self.kappa = self._m.E.shape[0]
         self.ita = self._m.I.shape[0]
@@ -405,9 +423,9 @@ class Job:
 
     def __init__(self, tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64, n_c=10,
                  eta=0.03, ssa_burst=100, t_max=1.0, I=None, E=(0,), n_sims=1, sims_per_set=None, seed=0,
-                 sim_offset=0, specialize=False, max_steps=0):
+                 sim_offset=0, specialize=False, max_steps=0, layout=LAYOUT_ROWS):
         self.m = _Marshalled(tables, rates, x0, method, precision, n_c, eta, ssa_burst, t_max, I, E, n_sims,
-                             sims_per_set, seed, sim_offset, specialize, max_steps)
+                             sims_per_set, seed, sim_offset, specialize, max_steps, layout)
         self.n_sims = int(n_sims)
         self.kappa = self.m.E.shape[0]
         self.ita = self.m.I.shape[0]

@@ -45,7 +45,7 @@ struct JobParams {
   int32_t n_rates;      // length of a rates row: M, or the parameter count of compiled hazards
   uint32_t max_steps;   // 0: none; else leap decisions + SSA steps per simulation
   int32_t lanes;        // simulations per wavefront, 1..64 (launch tuning)
-  int32_t reserved;
+  int32_t layout;       // trajectory layout of O: LAYOUT_ROWS or LAYOUT_BLOCKS (pst_driver.hpp)
   double eta;           // tl_parser.py:44
   double t_max;
   uint64_t seed;
@@ -54,7 +54,7 @@ struct JobParams {
   const double* rates;  // [n_sets][M]
   const int32_t* x0;    // [n_sets][N]
   const float* I;       // [ita] sample times
-  int32_t* O;           // [kappa][ita][o_stride]
+  int32_t* O;           // rows: [kappa][ita][o_stride]; blocks: [kappa][ceil(ita/8)][o_stride][8]
   int32_t* flags;       // [n_sims] or null
 };
 
@@ -89,6 +89,7 @@ struct JobRef {
   static constexpr int kNP = NP, kMP = MP;
   static constexpr int kKP = MP;          // rates per row: one per reaction
   static constexpr bool kCustom = false;  // propensities from the mass-action table
+  static constexpr int kOutLds = NP < 4 ? NP : 4;  // recorded species gathered in LDS (blocked layout)
   using Dev = JobDev<NP, MP>;
   static constexpr int NW = (NP * MP + 3) / 4;
   static constexpr int NS = (NP + 3) / 4;
@@ -161,6 +162,7 @@ struct FixedRef {
   static constexpr int kNP = Net::N, kMP = Net::M;
   static constexpr int kKP = Net::n_rates;   // rates per row
   static constexpr bool kCustom = Net::custom;  // compiled hazard expressions
+  static constexpr int kOutLds = Net::kappa < 4 ? Net::kappa : 4;  // recorded species gathered in LDS
   template <class R>
   __device__ __forceinline__ static void props(const int32_t (&x)[kNP], const R (&c)[kKP], R (&a)[kMP]) {
     Net::template props<R>(x, c, a);

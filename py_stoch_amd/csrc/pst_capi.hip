@@ -134,6 +134,28 @@ __global__ void zero_u64(unsigned long long* p, int n) {
   if (i < n) p[i] = 0ull;
 }
 
+// blocks[kappa][ita8][bstride][8] -> rows[kappa][ita][rstride]: workgroup b
+// takes block row r = b / chunks (species q = r / ita8, samples 8 (r % ita8)
+// ...) and 256 simulations; each lane reads its simulation's 32 bytes with
+// two 16-byte loads and writes up to eight rows, consecutive lanes to
+// consecutive addresses.
+__global__ void __launch_bounds__(256) blocks_to_rows(const int32_t* __restrict__ blocks, int64_t bstride,
+                                                      int32_t* __restrict__ rows, int64_t rstride, int ita,
+                                                      int ita8, int64_t n_sims, int64_t chunks) {
+  const int64_t b = blockIdx.x;
+  const int64_t r = b / chunks;
+  const int64_t s = (b % chunks) * 256 + threadIdx.x;
+  if (s >= n_sims) return;
+  const int q = (int)(r / ita8);
+  const int f0 = (int)(r % ita8) * 8;
+  const int4* src = reinterpret_cast<const int4*>(blocks + (r * bstride + s) * 8);
+  const int4 lo = src[0], hi = src[1];
+  const int32_t w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (f0 + j < ita) rows[((int64_t)q * ita + f0 + j) * rstride + s] = w[j];
+}
+
 // Check a pst_network / pst_config pair and build the dense host table.
 // host_rows = false: rates, x0 and cfg->I are device arrays, and their
 // values are not checked on the host.
@@ -515,6 +537,26 @@ const char* pst_last_error(void) { return g_err; }
 
 int pst_abi_version(void) { return PST_ABI_VERSION; }
 
+int pst_blocks_to_rows(const int32_t* blocks, int64_t blocks_stride, int32_t* rows, int64_t rows_stride,
+                       int32_t kappa, int32_t ita, int64_t n_sims, void* stream) {
+  g_err[0] = 0;
+  if (kappa < 1 || ita < 1 || n_sims < 0) return fail(PST_EINVAL, "need kappa >= 1, ita >= 1, n_sims >= 0");
+  if (blocks_stride < n_sims || rows_stride < n_sims)
+    return fail(PST_EINVAL, "strides (%lld, %lld) below n_sims %lld", (long long)blocks_stride, (long long)rows_stride,
+                (long long)n_sims);
+  if (n_sims == 0) return PST_OK;
+  if (!blocks || !rows) return fail(PST_EINVAL, "null trajectory array");
+  if (((uintptr_t)blocks & 15u) != 0) return fail(PST_EINVAL, "blocks array is not 16-byte aligned");
+  const int ita8 = (ita + 7) / 8;
+  const int64_t chunks = (n_sims + 255) / 256;
+  const int64_t grid = (int64_t)kappa * ita8 * chunks;
+  if (grid > 0x7fffffffLL) return fail(PST_ETOOLARGE, "%lld workgroups exceed one launch", (long long)grid);
+  hipLaunchKernelGGL(blocks_to_rows, dim3((unsigned int)grid), dim3(256), 0, (hipStream_t)stream, blocks,
+                     blocks_stride, rows, rows_stride, ita, ita8, n_sims, chunks);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? PST_OK : hip_fail(e, "blocks_to_rows launch");
+}
+
 void pst_max_network(int32_t* max_species, int32_t* max_reactions) {
   if (max_species) *max_species = kMaxSpecies;
   if (max_reactions) *max_reactions = kMaxReactions;
@@ -567,7 +609,8 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
       return fail(PST_EINVAL, "job %d has precision %d, job 0 has %d: one precision per batch", q,
                   J.cfg.precision, jobs[0].cfg.precision);
     if (J.cfg.specialize != 0 && J.cfg.specialize != 1) return fail(PST_EINVAL, "job %d: specialize must be 0 or 1", q);
-    if (J.cfg.reserved != 0) return fail(PST_EINVAL, "job %d: reserved config field must be 0", q);
+    if (J.cfg.layout != PST_LAYOUT_ROWS && J.cfg.layout != PST_LAYOUT_BLOCKS)
+      return fail(PST_EINVAL, "job %d: unknown trajectory layout %d", q, J.cfg.layout);
     const int kd = kernel_kind(J, nets[q]);
     if (kd < 0) return kd;
     kinds[q] = kd;
@@ -628,6 +671,7 @@ int pst_batch_plan_create(const pst_job* jobs, int32_t n_jobs, pst_plan** out) {
     P.lanes = 64;
     P.n_rates = n_rates;
     P.max_steps = (uint32_t)J.cfg.max_steps;
+    P.layout = J.cfg.layout;
     P.eta = J.cfg.eta;
     P.t_max = J.cfg.t_max;
     P.seed = J.cfg.seed;
@@ -1014,6 +1058,9 @@ int pst_plan_set_stream(pst_plan* p, uint64_t seed, uint64_t sim_offset) {
 int pst_batch_run(const pst_job* jobs, int32_t n_jobs, int32_t* const* O, int32_t* const* flags, pst_stats* stats) {
   g_err[0] = 0;
   if (!O) return fail(PST_EINVAL, "null trajectory array list");
+  for (int q = 0; jobs && q < n_jobs; ++q)
+    if (jobs[q].cfg.layout != PST_LAYOUT_ROWS)
+      return fail(PST_EINVAL, "job %d: host arrays are returned as rows (layout must be PST_LAYOUT_ROWS)", q);
   pst_plan* p = nullptr;
   int rc = pst_batch_plan_create(jobs, n_jobs, &p);
   if (rc != PST_OK) return rc;
@@ -1097,7 +1144,8 @@ int pst_run_noalloc(const pst_network* net, const pst_config* cfg, const double*
                             "need pst_plan_create");
   if (cfg->specialize != 0)
     return fail(PST_EINVAL, "pst_run_noalloc runs the library's table-driven kernels: specialize must be 0");
-  if (cfg->reserved != 0) return fail(PST_EINVAL, "reserved config field must be 0");
+  if (cfg->layout != PST_LAYOUT_ROWS && cfg->layout != PST_LAYOUT_BLOCKS)
+    return fail(PST_EINVAL, "unknown trajectory layout %d", cfg->layout);
   pst_config c = *cfg;
   c.I = I_dev;
   static thread_local pst::HostNet h;  // ~50 KB: kept off the stack, and no heap allocation
@@ -1139,6 +1187,7 @@ int pst_run_noalloc(const pst_network* net, const pst_config* cfg, const double*
   P.seed = c.seed;
   P.sim_offset = c.sim_offset;
   P.o_stride = o_stride;
+  P.layout = c.layout;
   P.rates = rates_dev;
   P.x0 = x0_dev;
   P.I = I_dev;

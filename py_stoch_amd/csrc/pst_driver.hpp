@@ -58,6 +58,84 @@ __device__ __forceinline__ int32_t pick_i(const int32_t (&x)[NP], int idx) {
   return r;
 }
 
+// ---------------------------------------------------------------------------
+// Trajectory output.  JobParams::layout selects
+//   LAYOUT_ROWS    O[q][f][o_stride], the reference's [kappa][ita][U]
+//                  (cuTauLeaping.py:126): one 4-byte store per sample.  Lanes
+//                  cross sample times at different moments, so every store
+//                  dirties a 32-byte sector of its own.
+//   LAYOUT_BLOCKS  O[q][f / 8][o_stride][f % 8] (ceil(ita / 8) blocks per
+//                  species): the eight consecutive samples of one
+//                  simulation fill one 32-byte sector.  A lane gathers its
+//                  simulation's samples of the first Ref::kOutLds recorded
+//                  species in LDS and writes each block with two 16-byte
+//                  stores once its last sample is known, so no sector is
+//                  written twice (further species are stored sample by
+//                  sample).  The LDS slot is per simulation lane of the
+//                  workgroup: the decide and burst lanes of a pair
+//                  (run_driver_paired) share it, so a simulation's
+//                  partial block travels with it.
+// The last block is padded with zeros.
+// ---------------------------------------------------------------------------
+constexpr int LAYOUT_ROWS = 0, LAYOUT_BLOCKS = 1;
+constexpr int OUT_BLOCK = 8;
+
+template <int KB>
+struct SampleOut {
+  static constexpr int KS = KB > 0 ? KB : 1;
+  static __device__ __forceinline__ int32_t* lds() {
+    __shared__ int32_t buf[KS * OUT_BLOCK * 256];
+    return buf;
+  }
+  static __device__ __forceinline__ int slot() { return (int)(((threadIdx.x >> 6) & 3u) * 64u + (threadIdx.x & 63u)); }
+  template <class P>
+  static __device__ __forceinline__ int64_t block_row(const P& p, int q, int f) {
+    return ((int64_t)q * ((p.ita + OUT_BLOCK - 1) / OUT_BLOCK) + (f / OUT_BLOCK)) * p.o_stride * OUT_BLOCK;
+  }
+  // the simulation's base pointer: O + sim (rows) or O + 8 sim (blocks)
+  template <class P>
+  static __device__ __forceinline__ int32_t* base(const P& p, int64_t sim) {
+    return p.O + (p.layout == LAYOUT_BLOCKS ? sim * OUT_BLOCK : sim);
+  }
+  // sample f of recorded species q
+  template <class P>
+  static __device__ __forceinline__ void put(const P& p, int32_t* ob, int q, int f, int32_t v) {
+    if (p.layout != LAYOUT_BLOCKS) {
+      ob[((int64_t)q * p.ita + f) * p.o_stride] = v;
+    } else if (q < KB) {
+      lds()[(q * OUT_BLOCK + (f & (OUT_BLOCK - 1))) * 256 + slot()] = v;
+    } else {
+      ob[block_row(p, q, f) + (f & (OUT_BLOCK - 1))] = v;
+    }
+  }
+  // after sample f of every recorded species: write the gathered blocks
+  // that sample f completes (the block's last sample, or the run's)
+  template <class P>
+  static __device__ __forceinline__ void row_done(const P& p, int32_t* ob, int kappa, int f) {
+    if (p.layout != LAYOUT_BLOCKS) return;
+    const int last = f & (OUT_BLOCK - 1);
+    if (last != OUT_BLOCK - 1 && f != p.ita - 1) return;
+#pragma unroll
+    for (int q = 0; q < KB; ++q) {
+      if (q < kappa) {
+        int32_t w[OUT_BLOCK];
+#pragma unroll
+        for (int j = 0; j < OUT_BLOCK; ++j) {
+          const int32_t v = lds()[(q * OUT_BLOCK + j) * 256 + slot()];
+          w[j] = (j <= last) ? v : 0;
+        }
+        int4* dst = reinterpret_cast<int4*>(ob + block_row(p, q, f));
+        dst[0] = make_int4(w[0], w[1], w[2], w[3]);
+        dst[1] = make_int4(w[4], w[5], w[6], w[7]);
+      }
+    }
+    // species stored sample by sample: zero the padding of the last block
+    if (f == p.ita - 1)
+      for (int q = KB; q < kappa; ++q)
+        for (int j = last + 1; j < OUT_BLOCK; ++j) ob[block_row(p, q, f) + j] = 0;
+  }
+};
+
 // Engine interface (see EngineF64 / EngineF32):
 //   int  start(job, sim)            -> initial mode, state loaded
 //   int  step<SSA_ONLY>(job, cnt, m) -> next mode (DONE may set flag)
@@ -330,6 +408,8 @@ __device__ __forceinline__ void run_driver_paired(const typename Ref::Dev* __res
     lds_store_release(&s_done[pair], 1u);
   } else {
     // ---- burst wavefront ----
+    // (Both roles keep the default issue priority: raising the burst
+    // wavefront's slowed the Lotka-Volterra workload from 39.9 to 42.3 ms.)
     bool busy = false;
     int bjob = -1;
     for (;;) {

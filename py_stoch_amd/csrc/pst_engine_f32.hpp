@@ -179,15 +179,26 @@ struct EngineF32 {
   uint32_t nsteps;  // leap decisions + SSA steps so far (step budget)
   int32_t* Ob;
 
+  using Out = SampleOut<Ref::kOutLds>;
+
   __device__ __forceinline__ void fill_rest(const Ref& tab, bool zero, Counters& cnt) const {
     const auto& P = tab.p();
     cnt.w.samples += (unsigned int)(P.ita - F);
+    if (P.layout != LAYOUT_BLOCKS) {
 #pragma unroll
-    for (int q = 0; q < NP; ++q) {
-      if (q < P.kappa) {
-        const int32_t val = zero ? 0 : pick_i<NP>(x, tab.Ev(q));
-        for (int f = F; f < P.ita; ++f) Ob[((int64_t)q * P.ita + f) * P.o_stride] = val;
+      for (int q = 0; q < NP; ++q) {
+        if (q < P.kappa) {
+          const int32_t val = zero ? 0 : pick_i<NP>(x, tab.Ev(q));
+          for (int f = F; f < P.ita; ++f) Ob[((int64_t)q * P.ita + f) * P.o_stride] = val;
+        }
       }
+      return;
+    }
+    for (int f = F; f < P.ita; ++f) {
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+        if (q < P.kappa) Out::put(P, Ob, q, f, zero ? 0 : pick_i<NP>(x, tab.Ev(q)));
+      Out::row_done(P, Ob, P.kappa, f);
     }
   }
 
@@ -206,7 +217,7 @@ struct EngineF32 {
     flag = 0;
     burst = 0;
     nsteps = 0;
-    Ob = P.O + sim;
+    Ob = Out::base(P, sim);
     return P.method == 0 ? MODE_DECIDE : MODE_SSA;
   }
 
@@ -409,9 +420,10 @@ struct EngineF32 {
             const int s = tab.Ev(q);
             const int32_t xs = pick_i<NP>(x, s);
             const int diff = pick_i<NP>(xn, s) - xs;
-            Ob[((int64_t)q * P.ita + F) * P.o_stride] = xs + rn_f32_to_int((float)diff * frac);
+            Out::put(P, Ob, q, F, xs + rn_f32_to_int((float)diff * frac));
           }
         }
+        Out::row_done(P, Ob, P.kappa, F);
         cnt.w.samples += 1;
         F = F + 1;
         if (F == P.ita) return MODE_DONE;

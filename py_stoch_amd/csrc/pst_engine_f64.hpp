@@ -142,17 +142,28 @@ struct EngineF64 {
   int flag;
   int burst;    // SSA steps left in the current fallback burst (method 0)
   uint32_t nsteps;  // leap decisions + SSA steps so far (step budget)
-  int32_t* Ob;  // &O[0][0][sim]
+  int32_t* Ob;  // the simulation's base in O (SampleOut::base)
+
+  using Out = SampleOut<Ref::kOutLds>;
 
   __device__ __forceinline__ void fill_rest(const Ref& tab, bool zero, Counters& cnt) const {
     const auto& P = tab.p();
     cnt.w.samples += (unsigned int)(P.ita - F);
+    if (P.layout != LAYOUT_BLOCKS) {
 #pragma unroll
-    for (int q = 0; q < NP; ++q) {
-      if (q < P.kappa) {
-        const int32_t val = zero ? 0 : pick_i<NP>(x, tab.Ev(q));
-        for (int f = F; f < P.ita; ++f) Ob[((int64_t)q * P.ita + f) * P.o_stride] = val;
+      for (int q = 0; q < NP; ++q) {
+        if (q < P.kappa) {
+          const int32_t val = zero ? 0 : pick_i<NP>(x, tab.Ev(q));
+          for (int f = F; f < P.ita; ++f) Ob[((int64_t)q * P.ita + f) * P.o_stride] = val;
+        }
       }
+      return;
+    }
+    for (int f = F; f < P.ita; ++f) {
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+        if (q < P.kappa) Out::put(P, Ob, q, f, zero ? 0 : pick_i<NP>(x, tab.Ev(q)));
+      Out::row_done(P, Ob, P.kappa, f);
     }
   }
 
@@ -171,7 +182,7 @@ struct EngineF64 {
     flag = 0;
     burst = 0;
     nsteps = 0;
-    Ob = P.O + sim;
+    Ob = Out::base(P, sim);
     return P.method == 0 ? MODE_DECIDE : MODE_SSA;
   }
 
@@ -358,9 +369,10 @@ struct EngineF64 {
           const int32_t xs = pick_i<NP>(x, s);
           const int diff = pick_i<NP>(xn, s) - xs;
           const double inc = (double)diff * frac;
-          Ob[((int64_t)q * P.ita + F) * P.o_stride] = ssa ? xs : xs + rn_f32_to_int((float)inc);
+          Out::put(P, Ob, q, F, ssa ? xs : xs + rn_f32_to_int((float)inc));
         }
       }
+      Out::row_done(P, Ob, P.kappa, F);
       cnt.w.samples += 1;
       F = F + 1;
       if (F == P.ita) return MODE_DONE;  // an SSA step ending the run is not applied

@@ -112,14 +112,29 @@ struct EngineWave {
   uint32_t nsteps;  // leap decisions + SSA steps so far (step budget)
   int64_t sim;
 
+  // index of sample f of recorded species q in O (rows or blocks layout,
+  // pst_driver.hpp; stored sample by sample by the species lane)
+  __device__ __forceinline__ int64_t out_index(const JobParams& P, int q, int f) const {
+    if (P.layout != LAYOUT_BLOCKS) return ((int64_t)q * P.ita + f) * P.o_stride + sim;
+    return (((int64_t)q * ((P.ita + OUT_BLOCK - 1) / OUT_BLOCK) + f / OUT_BLOCK) * P.o_stride + sim) * OUT_BLOCK +
+           (f & (OUT_BLOCK - 1));
+  }
+
+  // blocks layout: zeros after the last sample in its block
+  __device__ __forceinline__ void pad_last_block(const JobParams& P, int q) const {
+    if (P.layout != LAYOUT_BLOCKS) return;
+    const int end = (P.ita + OUT_BLOCK - 1) / OUT_BLOCK * OUT_BLOCK;
+    for (int f = P.ita; f < end; ++f) P.O[out_index(P, q, f)] = 0;
+  }
+
   // write samples F..ita-1: x of the recorded species, or zeros
   __device__ __forceinline__ void fill_rest(const WaveJob& J, const WaveLds& L, int lane, bool zero, Counters& cnt) {
     const JobParams& P = J.p;
     if (lane == 0) cnt.w.samples += (unsigned int)(P.ita - F);
     if (lane < P.kappa) {
       const int32_t val = zero ? 0 : L.x[J.t.E[lane]];
-      int32_t* O = P.O + sim;
-      for (int f = F; f < P.ita; ++f) O[((int64_t)lane * P.ita + f) * P.o_stride] = val;
+      for (int f = F; f < P.ita; ++f) P.O[out_index(P, lane, f)] = val;
+      pad_last_block(P, lane);
     }
   }
 
@@ -334,7 +349,8 @@ struct EngineWave {
           const int s = T.E[lane];
           const int32_t xs = L.x[s];
           const int diff = (int)(L.xn[s] - (long long)xs);
-          P.O[((int64_t)lane * P.ita + F) * P.o_stride + sim] = xs + rn_f32_to_int((float)diff * frac);
+          P.O[out_index(P, lane, F)] = xs + rn_f32_to_int((float)diff * frac);
+          if (F == P.ita - 1) pad_last_block(P, lane);
         }
         if (lane == 0) cnt.w.samples += 1;
         F = F + 1;
