@@ -27,6 +27,18 @@ using __hip_internal::uint64_t;
 
 namespace pst {
 
+// Global-address-space view of a pointer held in a generic register.  Loads
+// and stores through it are global_* instructions, counted by vmcnt only;
+// through the generic pointer they are flat_* instructions, which count in
+// lgkmcnt too, so every wait for an LDS result (handoff slots, sample blocks)
+// would also wait for the trajectory stores and sample-time loads in flight.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> glb(T* p) {
+  return (gptr<T>)p;
+}
+
 // ---------------------------------------------------------------------------
 // limits shared with the host library (flag bits are part of the C ABI)
 // ---------------------------------------------------------------------------

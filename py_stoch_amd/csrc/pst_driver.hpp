@@ -79,6 +79,7 @@ __device__ __forceinline__ int32_t pick_i(const int32_t (&x)[NP], int idx) {
 // ---------------------------------------------------------------------------
 constexpr int LAYOUT_ROWS = 0, LAYOUT_BLOCKS = 1;
 constexpr int OUT_BLOCK = 8;
+typedef int32_t I4 __attribute__((ext_vector_type(4)));  // one 16-byte store
 
 template <int KB>
 struct SampleOut {
@@ -101,11 +102,11 @@ struct SampleOut {
   template <class P>
   static __device__ __forceinline__ void put(const P& p, int32_t* ob, int q, int f, int32_t v) {
     if (p.layout != LAYOUT_BLOCKS) {
-      ob[((int64_t)q * p.ita + f) * p.o_stride] = v;
+      glb(ob)[((int64_t)q * p.ita + f) * p.o_stride] = v;
     } else if (q < KB) {
       lds()[(q * OUT_BLOCK + (f & (OUT_BLOCK - 1))) * 256 + slot()] = v;
     } else {
-      ob[block_row(p, q, f) + (f & (OUT_BLOCK - 1))] = v;
+      glb(ob)[block_row(p, q, f) + (f & (OUT_BLOCK - 1))] = v;
     }
   }
   // after sample f of every recorded species: write the gathered blocks
@@ -124,15 +125,15 @@ struct SampleOut {
           const int32_t v = lds()[(q * OUT_BLOCK + j) * 256 + slot()];
           w[j] = (j <= last) ? v : 0;
         }
-        int4* dst = reinterpret_cast<int4*>(ob + block_row(p, q, f));
-        dst[0] = make_int4(w[0], w[1], w[2], w[3]);
-        dst[1] = make_int4(w[4], w[5], w[6], w[7]);
+        gptr<I4> dst = (gptr<I4>)glb(ob + block_row(p, q, f));
+        dst[0] = I4{w[0], w[1], w[2], w[3]};
+        dst[1] = I4{w[4], w[5], w[6], w[7]};
       }
     }
     // species stored sample by sample: zero the padding of the last block
     if (f == p.ita - 1)
       for (int q = KB; q < kappa; ++q)
-        for (int j = last + 1; j < OUT_BLOCK; ++j) ob[block_row(p, q, f) + j] = 0;
+        for (int j = last + 1; j < OUT_BLOCK; ++j) glb(ob)[block_row(p, q, f) + j] = 0;
   }
 };
 
@@ -194,7 +195,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
         mode = eng.template step<false>(jr, cnt, mode);
         if (mode == MODE_DONE) {
           int32_t* fl = jr.p().flags;
-          if (fl) fl[sim] = eng.flag;
+          if (fl) glb(fl)[sim] = eng.flag;
           cnt.flagged += (eng.flag != 0);
           cnt.fevents += (eng.flag != 0) ? cnt.events - ev_start : 0ull;
         }
@@ -208,7 +209,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
           mode = eng.template step<true>(jr, cnt, mode);
           if (mode == MODE_DONE) {
             int32_t* fl = jr.p().flags;
-            if (fl) fl[sim] = eng.flag;
+            if (fl) glb(fl)[sim] = eng.flag;
             cnt.flagged += (eng.flag != 0);
             cnt.fevents += (eng.flag != 0) ? cnt.events - ev_start : 0ull;
           }
@@ -226,7 +227,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
           mode = eng.template step<true>(jr, cnt, mode);
           if (mode == MODE_DONE) {
             int32_t* fl = jr.p().flags;
-            if (fl) fl[sim] = eng.flag;
+            if (fl) glb(fl)[sim] = eng.flag;
             cnt.flagged += (eng.flag != 0);
             cnt.fevents += (eng.flag != 0) ? cnt.events - ev_start : 0ull;
           }
@@ -320,7 +321,7 @@ template <class Engine, class Ref>
 __device__ __forceinline__ void finish_sim(const Ref& jr, const Engine& eng, int64_t sim, Counters& cnt,
                                            unsigned long long ev_start) {
   int32_t* fl = jr.p().flags;
-  if (fl) fl[sim] = eng.flag;
+  if (fl) glb(fl)[sim] = eng.flag;
   cnt.flagged += (eng.flag != 0);
   cnt.fevents += (eng.flag != 0) ? cnt.events - ev_start : 0ull;
 }

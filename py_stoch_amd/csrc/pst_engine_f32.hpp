@@ -189,7 +189,7 @@ struct EngineF32 {
       for (int q = 0; q < NP; ++q) {
         if (q < P.kappa) {
           const int32_t val = zero ? 0 : pick_i<NP>(x, tab.Ev(q));
-          for (int f = F; f < P.ita; ++f) Ob[((int64_t)q * P.ita + f) * P.o_stride] = val;
+          for (int f = F; f < P.ita; ++f) glb(Ob)[((int64_t)q * P.ita + f) * P.o_stride] = val;
         }
       }
       return;
@@ -206,14 +206,14 @@ struct EngineF32 {
     const auto& P = jr.p();
     const int64_t set = sim / P.sims_per_set;
 #pragma unroll
-    for (int j = 0; j < Ref::kKP; ++j) k[j] = (j < P.n_rates) ? (float)P.rates[set * P.n_rates + j] : 0.0f;
+    for (int j = 0; j < Ref::kKP; ++j) k[j] = (j < P.n_rates) ? (float)glb(P.rates)[set * P.n_rates + j] : 0.0f;
 #pragma unroll
-    for (int i = 0; i < NP; ++i) x[i] = (i < P.N) ? P.x0[set * P.N + i] : 0;
+    for (int i = 0; i < NP; ++i) x[i] = (i < P.N) ? glb(P.x0)[set * P.N + i] : 0;
     key.init(P.seed, P.sim_offset + (uint64_t)sim);
     bc = 0;
     t = 0.0;
     F = 0;
-    tnext = (double)P.I[0];
+    tnext = (double)glb(P.I)[0];
     flag = 0;
     burst = 0;
     nsteps = 0;
@@ -427,7 +427,7 @@ struct EngineF32 {
         cnt.w.samples += 1;
         F = F + 1;
         if (F == P.ita) return MODE_DONE;
-        tnext = (double)P.I[F];
+        tnext = (double)glb(P.I)[F];
       }
     }
     if (ssa) {

@@ -154,7 +154,7 @@ struct EngineF64 {
       for (int q = 0; q < NP; ++q) {
         if (q < P.kappa) {
           const int32_t val = zero ? 0 : pick_i<NP>(x, tab.Ev(q));
-          for (int f = F; f < P.ita; ++f) Ob[((int64_t)q * P.ita + f) * P.o_stride] = val;
+          for (int f = F; f < P.ita; ++f) glb(Ob)[((int64_t)q * P.ita + f) * P.o_stride] = val;
         }
       }
       return;
@@ -171,14 +171,14 @@ struct EngineF64 {
     const auto& P = jr.p();
     const int64_t set = sim / P.sims_per_set;
 #pragma unroll
-    for (int j = 0; j < Ref::kKP; ++j) k[j] = (j < P.n_rates) ? P.rates[set * P.n_rates + j] : 0.0;
+    for (int j = 0; j < Ref::kKP; ++j) k[j] = (j < P.n_rates) ? glb(P.rates)[set * P.n_rates + j] : 0.0;
 #pragma unroll
-    for (int i = 0; i < NP; ++i) x[i] = (i < P.N) ? P.x0[set * P.N + i] : 0;
+    for (int i = 0; i < NP; ++i) x[i] = (i < P.N) ? glb(P.x0)[set * P.N + i] : 0;
     key.init(P.seed, P.sim_offset + (uint64_t)sim);
     bc = 0;
     t = 0.0;
     F = 0;
-    tnext = (double)P.I[0];
+    tnext = (double)glb(P.I)[0];
     flag = 0;
     burst = 0;
     nsteps = 0;
@@ -358,7 +358,7 @@ struct EngineF64 {
     }
     while (t >= tnext) {
       const double t0 = t - tau;
-      const float record_time = P.I[F];
+      const float record_time = glb(P.I)[F];
       const double gap = t - t0;
       const double nrt = (double)record_time - t0;
       const double frac = nrt / gap;
@@ -376,7 +376,7 @@ struct EngineF64 {
       cnt.w.samples += 1;
       F = F + 1;
       if (F == P.ita) return MODE_DONE;  // an SSA step ending the run is not applied
-      tnext = (double)P.I[F];
+      tnext = (double)glb(P.I)[F];
     }
     if (ssa) {
       bool over = false;

@@ -124,7 +124,7 @@ struct EngineWave {
   __device__ __forceinline__ void pad_last_block(const JobParams& P, int q) const {
     if (P.layout != LAYOUT_BLOCKS) return;
     const int end = (P.ita + OUT_BLOCK - 1) / OUT_BLOCK * OUT_BLOCK;
-    for (int f = P.ita; f < end; ++f) P.O[out_index(P, q, f)] = 0;
+    for (int f = P.ita; f < end; ++f) glb(P.O)[out_index(P, q, f)] = 0;
   }
 
   // write samples F..ita-1: x of the recorded species, or zeros
@@ -133,7 +133,7 @@ struct EngineWave {
     if (lane == 0) cnt.w.samples += (unsigned int)(P.ita - F);
     if (lane < P.kappa) {
       const int32_t val = zero ? 0 : L.x[J.t.E[lane]];
-      for (int f = F; f < P.ita; ++f) P.O[out_index(P, lane, f)] = val;
+      for (int f = F; f < P.ita; ++f) glb(P.O)[out_index(P, lane, f)] = val;
       pad_last_block(P, lane);
     }
   }
@@ -145,15 +145,15 @@ struct EngineWave {
 #pragma unroll
     for (int r = 0; r < WAVE_R; ++r) {
       const int j = lane + 64 * r;
-      k[r] = (j < P.M) ? (float)P.rates[set * P.M + j] : 0.0f;
+      k[r] = (j < P.M) ? (float)glb(P.rates)[set * P.M + j] : 0.0f;
     }
-    if (lane < P.N) L.x[lane] = P.x0[set * P.N + lane];
+    if (lane < P.N) L.x[lane] = glb(P.x0)[set * P.N + lane];
     wave_lds_sync();
     key.init(P.seed, P.sim_offset + (uint64_t)s);
     bc = 0;
     t = 0.0;
     F = 0;
-    tnext = (double)P.I[0];
+    tnext = (double)glb(P.I)[0];
     flag = 0;
     burst = 0;
     nsteps = 0;
@@ -349,13 +349,13 @@ struct EngineWave {
           const int s = T.E[lane];
           const int32_t xs = L.x[s];
           const int diff = (int)(L.xn[s] - (long long)xs);
-          P.O[out_index(P, lane, F)] = xs + rn_f32_to_int((float)diff * frac);
+          glb(P.O)[out_index(P, lane, F)] = xs + rn_f32_to_int((float)diff * frac);
           if (F == P.ita - 1) pad_last_block(P, lane);
         }
         if (lane == 0) cnt.w.samples += 1;
         F = F + 1;
         if (F == P.ita) return true;  // an SSA step ending the run is not applied
-        tnext = (double)P.I[F];
+        tnext = (double)glb(P.I)[F];
       }
     }
     const bool over = __builtin_amdgcn_ballot_w64(lane < N && L.xn[lane] >= COUNT_LIMIT) != 0ull;
