@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define PST_ABI_VERSION 6
+#define PST_ABI_VERSION 7
 
 /* status codes */
 #define PST_OK 0
@@ -258,6 +258,37 @@ int pst_plan_autotune(pst_plan* plan, int32_t* const* O_dev, const int64_t* o_st
  * burst_waves 0 or 1.  No reference counterpart. */
 int pst_plan_set_launch(pst_plan* plan, int32_t lanes_shift, int32_t ssa_slice, int32_t waves_trim,
                         int32_t burst_waves);
+
+/*
+ * Sample ranges and graph-captured chunks.  The reference's host loop
+ * (cuTauLeaping.py:48-76) relaunches its kernels until every simulation is
+ * done; here one launch normally runs every simulation to its end.  These
+ * calls split a run at sample boundaries instead, for progress, checkpoints
+ * (between ranges the plan's device memory holds every simulation's state)
+ * and HIP graph capture; the results are bit-identical to one execute.
+ * pst_plan_execute_range runs every simulation from where the plan's
+ * previous range stopped (from the start if no range is pending) until it
+ * has recorded samples 0 .. f_end - 1 of O_dev (or ended); once f_end
+ * reaches every job's last sample the run is complete and the next range
+ * starts afresh.  Jobs writing PST_LAYOUT_BLOCKS need f_end to be a multiple
+ * of 8 (or the last sample).  Large-network jobs are not supported.
+ * pst_plan_execute and pst_plan_autotune abandon a pending range sequence.
+ * pst_plan_range_position reports the samples recorded so far (0: none
+ * pending).  pst_plan_graph_create captures the fill kernels and launches of
+ * n_ranges consecutive ranges (f_ends increasing, the last reaching every
+ * job's last sample) into a HIP graph with the given buffers;
+ * pst_plan_graph_launch replays it on `stream` as one whole run (counters
+ * and kernel time over all ranges).  A graph replays the plan as captured:
+ * after pst_plan_set_launch, pst_plan_set_stream or an execute that changed
+ * the plan's descriptors (other buffers) it must be captured again
+ * (pst_plan_graph_launch then fails).  No reference counterpart.
+ */
+int pst_plan_execute_range(pst_plan* plan, int32_t* const* O_dev, const int64_t* o_strides,
+                           int32_t* const* flags_dev, int32_t f_end, void* stream);
+int pst_plan_range_position(pst_plan* plan, int32_t* samples_done);
+int pst_plan_graph_create(pst_plan* plan, int32_t* const* O_dev, const int64_t* o_strides,
+                          int32_t* const* flags_dev, const int32_t* f_ends, int32_t n_ranges);
+int pst_plan_graph_launch(pst_plan* plan, void* stream);
 
 /*
  * No counterpart in the reference: for callers that cannot allocate device

@@ -12,7 +12,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libpst_hip.so")
-ABI_VERSION = 6  # PST_ABI_VERSION of include/pst.h
+ABI_VERSION = 7  # PST_ABI_VERSION of include/pst.h
 
 METHOD_TAU_LEAPING = 0
 METHOD_FORCED_SSA = 1
@@ -144,6 +144,10 @@ EXPORTED_SYMBOLS = (
     "pst_plan_autotune",
     "pst_plan_set_launch",
     "pst_blocks_to_rows",
+    "pst_plan_execute_range",
+    "pst_plan_range_position",
+    "pst_plan_graph_create",
+    "pst_plan_graph_launch",
 )
 
 _lib = None
@@ -188,6 +192,17 @@ def load():
     L.pst_blocks_to_rows.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
                                      ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p]
     L.pst_blocks_to_rows.restype = ctypes.c_int
+    _P_VP = ctypes.POINTER(ctypes.c_void_p)
+    L.pst_plan_execute_range.argtypes = [ctypes.c_void_p, _P_VP, ctypes.POINTER(ctypes.c_int64), _P_VP,
+                                         ctypes.c_int32, ctypes.c_void_p]
+    L.pst_plan_execute_range.restype = ctypes.c_int
+    L.pst_plan_range_position.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+    L.pst_plan_range_position.restype = ctypes.c_int
+    L.pst_plan_graph_create.argtypes = [ctypes.c_void_p, _P_VP, ctypes.POINTER(ctypes.c_int64), _P_VP,
+                                        ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
+    L.pst_plan_graph_create.restype = ctypes.c_int
+    L.pst_plan_graph_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.pst_plan_graph_launch.restype = ctypes.c_int
     L.pst_plan_create.argtypes = [ctypes.POINTER(PstNetwork), ctypes.POINTER(PstConfig), _P_F64,
                                   _P_I32, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
     L.pst_plan_create.restype = ctypes.c_int
@@ -346,6 +361,40 @@ def poisson_probe_mixed(lam, words):
     return k, path
 
 
+def _ptr_arrays(O_ptrs, strides, flag_ptrs):
+    n = len(O_ptrs)
+    Op = (ctypes.c_void_p * n)(*[int(p) for p in O_ptrs])
+    Sp = (ctypes.c_int64 * n)(*[int(s) for s in strides])
+    Fp = (ctypes.c_void_p * n)(*[int(p) if p else None for p in flag_ptrs]) if flag_ptrs else None
+    return Op, Sp, Fp
+
+
+class _RangeMixin:
+    """Sample ranges and graph-captured chunks (pst_plan_execute_range,
+    pst_plan_graph_*); O_ptrs / strides / flag_ptrs as for a batch execute
+    (one entry per job)."""
+
+    def execute_range(self, O_ptrs, strides, f_end, flag_ptrs=None, stream_ptr=None):
+        Op, Sp, Fp = _ptr_arrays(O_ptrs, strides, flag_ptrs)
+        _check(load().pst_plan_execute_range(self._h, Op, Sp, Fp, int(f_end),
+                                             ctypes.c_void_p(int(stream_ptr)) if stream_ptr else None),
+               "pst_plan_execute_range")
+
+    def range_position(self):
+        out = ctypes.c_int32()
+        _check(load().pst_plan_range_position(self._h, ctypes.byref(out)), "pst_plan_range_position")
+        return int(out.value)
+
+    def graph_create(self, O_ptrs, strides, f_ends, flag_ptrs=None):
+        Op, Sp, Fp = _ptr_arrays(O_ptrs, strides, flag_ptrs)
+        ends = (ctypes.c_int32 * len(f_ends))(*[int(f) for f in f_ends])
+        _check(load().pst_plan_graph_create(self._h, Op, Sp, Fp, ends, len(f_ends)), "pst_plan_graph_create")
+
+    def graph_launch(self, stream_ptr=None):
+        _check(load().pst_plan_graph_launch(self._h, ctypes.c_void_p(int(stream_ptr)) if stream_ptr else None),
+               "pst_plan_graph_launch")
+
+
 def _autotune(h, O_ptrs, strides, flag_ptrs, stream_ptr, fraction):
     n = len(O_ptrs)
     Op = (ctypes.c_void_p * n)(*[int(p) for p in O_ptrs])
@@ -357,7 +406,7 @@ def _autotune(h, O_ptrs, strides, flag_ptrs, stream_ptr, fraction):
     return out.as_dict()
 
 
-class Plan:
+class Plan(_RangeMixin):
     """Device-resident run: tables uploaded once, trajectories left in device memory."""
 
     def __init__(self, tables, rates, x0, method=METHOD_TAU_LEAPING, precision=PRECISION_FP64,
@@ -456,7 +505,7 @@ def batch_simulate(jobs):
     return Os, [f[: j.n_sims] for f, j in zip(Fs, jobs)], st.as_dict()
 
 
-class BatchPlan:
+class BatchPlan(_RangeMixin):
     """Device-resident batch: tables uploaded once; execute() takes device
     addresses of one trajectory array (and optionally one flag array) per job."""
 

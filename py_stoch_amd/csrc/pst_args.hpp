@@ -56,7 +56,18 @@ struct JobParams {
   const float* I;       // [ita] sample times
   int32_t* O;           // rows: [kappa][ita][o_stride]; blocks: [kappa][ceil(ita/8)][o_stride][8]
   int32_t* flags;       // [n_sims] or null
+  uint32_t* state;      // [state_words][n_sims] saved simulations between sample ranges, or null
+  int32_t state_words;  // words per simulation in state (at least the engine's record)
+  int32_t reserved2;
 };
+
+// The sample range of a launch (pst_driver.hpp): stop simulations at sample
+// f_end (saving their state), and/or resume them from their saved state.
+struct Range {
+  int32_t f_end;
+  int32_t resume;
+};
+constexpr Range FULL_RUN = {0x7fffffff, 0};
 
 // Copy a job's settings into VGPRs (an empty asm on each 32-bit word hides
 // their uniformity).  They are read only by per-lane arithmetic (addresses,
@@ -220,9 +231,9 @@ inline NetTab<NP, MP> make_tab(const HostNet& h) {
 // of JobDev<NP,MP> for the size class `cls`, `queue` n_jobs zeroed counters,
 // `waves` the number of wavefronts to launch.
 int launch_fp64(int cls, const void* jobs, int n_jobs, unsigned int* queue, unsigned long long* stats,
-                int waves, hipStream_t stream);
+                int waves, Range range, hipStream_t stream);
 int launch_mixed(int cls, const void* jobs, int n_jobs, unsigned int* queue, unsigned long long* stats,
-                 int waves, hipStream_t stream);
+                 int waves, Range range, hipStream_t stream);
 // the mixed-precision Poisson sampler on given means and words (validation)
 int launch_poisson_probe(const float* lam, const uint32_t* words, int64_t n, int32_t* k, int32_t* path,
                          hipStream_t stream);

@@ -280,6 +280,8 @@ struct PlanJob {
   double* d_rates = nullptr;
   int32_t* d_x0 = nullptr;
   float* d_I = nullptr;
+  uint32_t* d_state = nullptr;  // saved simulations between sample ranges (allocated on first use)
+  int32_t state_words = 0;
 };
 
 template <int NP, int MP>
@@ -404,6 +406,11 @@ struct pst_plan {
   int ssa_slice = kSsaSlice;
   int waves_trim = 0;
   int burst_waves = 0;        // 1: tau-leaping network kernels run bursts on paired wavefronts
+  int range_pos = 0;          // samples every simulation has recorded in the pending ranges (0: none)
+  uint64_t generation = 0;    // changes whenever the descriptors or the launch shape change
+  hipGraph_t graph = nullptr; // sample-range chunks captured by pst_plan_graph_create
+  hipGraphExec_t graph_exec = nullptr;
+  uint64_t graph_generation = 0;
   // extra streams for plans with several kernel launches (created on first need)
   hipStream_t side[kSideStreams] = {};
   hipEvent_t fork = nullptr;
@@ -430,7 +437,8 @@ static hipError_t sync_plan(pst_plan* p, hipStream_t s) {
 // The kernel launches of one execute: size classes, network groups, then the
 // large-network kernel, each on the stream next_stream() gives.
 template <class NextStream>
-static int enqueue_launches(pst_plan* p, const int* count, int n_wave, int64_t wave_sims, NextStream& next_stream) {
+static int enqueue_launches(pst_plan* p, const int* count, int n_wave, int64_t wave_sims, pst::Range range,
+                            NextStream& next_stream) {
   hipError_t e;
   int qoff = 0;
   for (int c = 0; c < pst::NUM_CLASSES; ++c) {
@@ -444,8 +452,8 @@ static int enqueue_launches(pst_plan* p, const int* count, int n_wave, int64_t w
     const int waves = (int)(need < resident ? need : resident);
     // the queue counters of this class's jobs are queue[qoff .. qoff + count)
     const int rc = (p->precision == PST_PRECISION_MIXED)
-                       ? pst::launch_mixed(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, ls)
-                       : pst::launch_fp64(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, ls);
+                       ? pst::launch_mixed(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, range, ls)
+                       : pst::launch_fp64(c, p->d_jobs[c], count[c], p->d_queue + qoff, p->d_stats, waves, range, ls);
     if (rc < 0) return fail(PST_ETOOLARGE, "no compiled kernel for size class %d", c);
     if (rc != 0) return hip_fail((hipError_t)rc, "simulation kernel launch");
     qoff += count[c];
@@ -458,7 +466,8 @@ static int enqueue_launches(pst_plan* p, const int* count, int n_wave, int64_t w
     const int waves = (int)(need < resident ? need : resident);
     const int cnt = (int)g.jobs.size();
     const hipStream_t ls = next_stream();
-    if ((e = pst::rtc_launch(*k, (const pst::JobParams*)g.d_jobs, cnt, p->d_queue + qoff, p->d_stats, waves, ls)) !=
+    if ((e = pst::rtc_launch(*k, (const pst::JobParams*)g.d_jobs, cnt, p->d_queue + qoff, p->d_stats, waves, range,
+                             ls)) !=
         hipSuccess)
       return hip_fail(e, "specialised kernel launch");
     qoff += cnt;
@@ -583,6 +592,10 @@ void pst_plan_destroy(pst_plan* p) {
     if (p->join[i]) (void)hipEventDestroy(p->join[i]);
   }
   if (p->fork) (void)hipEventDestroy(p->fork);
+  for (PlanJob& j : p->jobs)
+    if (j.d_state) (void)hipFree(j.d_state);
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
   delete p;
 }
 
@@ -725,10 +738,13 @@ int pst_plan_create(const pst_network* net, const pst_config* cfg, const double*
   return pst_batch_plan_create(&j, 1, out);
 }
 
-int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_strides, int32_t* const* flags_dev,
-                           void* stream) {
-  g_err[0] = 0;
-  if (!p) return fail(PST_EINVAL, "null plan");
+// Everything an execute needs before its launches: buffer pointers, launch
+// shape and kernel builds per group, and the job descriptors in device
+// memory (uploaded, after the plan's earlier launches, only when they
+// changed).  with_state: the descriptors point at the jobs' saved-state
+// buffers (sample ranges).
+static int prepare_plan(pst_plan* p, int32_t* const* O_dev, const int64_t* o_strides, int32_t* const* flags_dev,
+                        hipStream_t s, bool with_state, int* count, int* n_wave_out, int64_t* wave_sims_out) {
   if (!O_dev || !o_strides) return fail(PST_EINVAL, "null trajectory buffer list");
   int dev = -1;
   (void)hipGetDevice(&dev);
@@ -742,8 +758,9 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
     pj.p.O = O_dev[q];
     pj.p.o_stride = o_strides[q];
     pj.p.flags = flags_dev ? flags_dev[q] : nullptr;
+    pj.p.state = with_state ? pj.d_state : nullptr;
+    pj.p.state_words = with_state ? pj.state_words : 0;
   }
-  hipStream_t s = (hipStream_t)stream;
   hipError_t e;
   // simulations per wavefront, per launch, and SSA-only steps per pass
   const int simds = p->cu_count * 4;
@@ -788,7 +805,7 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
     g.use_pair = true;
   }
   // job descriptors, one device array per size class
-  int count[pst::NUM_CLASSES] = {0, 0, 0};
+  for (int c = 0; c < pst::NUM_CLASSES; ++c) count[c] = 0;
   for (int c = 0; c < pst::NUM_CLASSES; ++c) {
     std::vector<unsigned char> bytes;
     if (c == 0) pack_jobs<4, 4>(p->jobs, c, bytes, &count[c]);
@@ -808,6 +825,7 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
     if ((e = hipMemcpy(p->d_jobs[c], bytes.data(), bytes.size(), hipMemcpyHostToDevice)) != hipSuccess)
       return hip_fail(e, "hipMemcpy(jobs)");
     p->uploaded[c] = bytes;
+    p->generation += 1;
   }
   // parameter blocks of the specialised kernels
   for (FixedGroup& g : p->fixed) {
@@ -827,6 +845,7 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
     if ((e = hipMemcpy(g.d_jobs, bytes.data(), bytes.size(), hipMemcpyHostToDevice)) != hipSuccess)
       return hip_fail(e, "hipMemcpy(jobs)");
     g.uploaded = bytes;
+    p->generation += 1;
   }
   // large-network jobs: parameters and sparse table per job
   int n_wave = 0;
@@ -855,14 +874,28 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
       if ((e = hipMemcpy(p->d_wave, bytes.data(), bytes.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "hipMemcpy(jobs)");
       p->uploaded_wave = bytes;
+      p->generation += 1;
     }
   }
+  *n_wave_out = n_wave;
+  *wave_sims_out = wave_sims;
+  return PST_OK;
+}
+
+// The launches of one execute (or one sample range) on stream s: the queue
+// counters (and, unless accumulate, the event counters) cleared by fill
+// kernels, the kernel launches spread over s and the side streams, ev0 / ev1
+// around them.
+static int enqueue_plan(pst_plan* p, hipStream_t s, const int* count, int n_wave, int64_t wave_sims,
+                        pst::Range range, bool accumulate) {
+  hipError_t e;
+  const int n_jobs = (int)p->jobs.size();
   // The queue and counters are shared by every execute of the plan: the
   // previous one, on whichever stream it was given, must be done with them
   // before they are cleared.
   p->executed = false;
   if (p->enqueued && (e = hipStreamWaitEvent(s, p->ev1, 0)) != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
-  hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, s, p->d_stats, (int)pst::ST_COUNT);
+  if (!accumulate) hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, s, p->d_stats, (int)pst::ST_COUNT);
   hipLaunchKernelGGL(zero_u32, dim3((n_jobs + 255) / 256), dim3(256), 0, s, p->d_queue, n_jobs);
   if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "zero kernels");
   if ((e = hipEventRecord(p->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
@@ -889,7 +922,7 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
     const int k = (li++) % (n_side + 1);
     return k == 0 ? s : p->side[k - 1];
   };
-  const int rc_launch = enqueue_launches(p, count, n_wave, wave_sims, next_stream);
+  const int rc_launch = enqueue_launches(p, count, n_wave, wave_sims, range, next_stream);
   // join the side streams back into the caller's stream and mark the end of
   // this execute, after a failed launch too: the next execute and
   // pst_plan_read_stats wait for ev1
@@ -903,6 +936,178 @@ int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_
   if (rc_launch != PST_OK) return rc_launch;
   if (ej != hipSuccess) return hip_fail(ej, "hipEventRecord / hipStreamWaitEvent");
   p->executed = true;
+  return PST_OK;
+}
+
+int pst_batch_plan_execute(pst_plan* p, int32_t* const* O_dev, const int64_t* o_strides, int32_t* const* flags_dev,
+                           void* stream) {
+  g_err[0] = 0;
+  if (!p) return fail(PST_EINVAL, "null plan");
+  hipStream_t s = (hipStream_t)stream;
+  int count[pst::NUM_CLASSES];
+  int n_wave = 0;
+  int64_t wave_sims = 0;
+  int rc = prepare_plan(p, O_dev, o_strides, flags_dev, s, false, count, &n_wave, &wave_sims);
+  if (rc != PST_OK) return rc;
+  p->range_pos = 0;  // a whole run abandons any pending sample ranges
+  return enqueue_plan(p, s, count, n_wave, wave_sims, pst::FULL_RUN, false);
+}
+
+// Saved-state buffers for sample ranges: the words one simulation's record
+// can take (the engine registers for the job's kernel -- rates, counts,
+// stream key, block counter, clock, sample cursor, burst and step counters,
+// trajectory pointer -- plus mode, events and the blocked layout's gathered
+// samples), a bound over both precisions.
+static int alloc_state(pst_plan* p) {
+  for (PlanJob& pj : p->jobs) {
+    if (pj.wave)
+      return fail(PST_EINVAL, "sample ranges run the lane kernels only; a job of %d species and %d reactions "
+                              "takes the large-network kernel", pj.net.N, pj.net.M);
+    if (pj.d_state) continue;
+    const int np = pj.cls >= 0 ? pst::CLASS_NP[pj.cls] : pj.net.N;
+    const int kp = pj.cls >= 0 ? pst::CLASS_MP[pj.cls] : std::max(1, pj.p.n_rates);
+    pj.state_words = 2 * kp + np + 32 + 8 * 4;  // + the gathered sample blocks of up to four species
+    const size_t bytes = sizeof(uint32_t) * (size_t)pj.state_words * (size_t)std::max(1, pj.p.n_sims);
+    hipError_t e = hipMalloc(&pj.d_state, bytes);
+    if (e != hipSuccess) {
+      pj.d_state = nullptr;
+      return hip_fail(e, "hipMalloc(saved simulation states)");
+    }
+  }
+  return PST_OK;
+}
+
+static int range_args_ok(pst_plan* p, int32_t f_end, int from) {
+  if (f_end <= from) return fail(PST_EINVAL, "range end %d must exceed %d, the samples already recorded", f_end, from);
+  for (size_t q = 0; q < p->jobs.size(); ++q) {
+    const pst::JobParams& P = p->jobs[q].p;
+    if (P.layout == PST_LAYOUT_BLOCKS && f_end < P.ita && f_end % 8 != 0)
+      return fail(PST_EINVAL, "job %zu writes the blocked layout: a range must end on a multiple of 8 samples "
+                              "or at the last sample (%d)", q, P.ita);
+  }
+  return PST_OK;
+}
+
+static int32_t max_ita(const pst_plan* p) {
+  int32_t m = 0;
+  for (const PlanJob& pj : p->jobs) m = std::max(m, pj.p.ita);
+  return m;
+}
+
+int pst_plan_execute_range(pst_plan* p, int32_t* const* O_dev, const int64_t* o_strides, int32_t* const* flags_dev,
+                           int32_t f_end, void* stream) {
+  g_err[0] = 0;
+  if (!p) return fail(PST_EINVAL, "null plan");
+  int rc = range_args_ok(p, f_end, p->range_pos);
+  if (rc == PST_OK) rc = alloc_state(p);
+  if (rc != PST_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  int count[pst::NUM_CLASSES];
+  int n_wave = 0;
+  int64_t wave_sims = 0;
+  rc = prepare_plan(p, O_dev, o_strides, flags_dev, s, true, count, &n_wave, &wave_sims);
+  if (rc != PST_OK) return rc;
+  const pst::Range r = {f_end, p->range_pos > 0 ? 1 : 0};
+  rc = enqueue_plan(p, s, count, n_wave, wave_sims, r, false);
+  if (rc != PST_OK) return rc;
+  p->range_pos = f_end >= max_ita(p) ? 0 : f_end;
+  return PST_OK;
+}
+
+int pst_plan_range_position(pst_plan* p, int32_t* samples_done) {
+  g_err[0] = 0;
+  if (!p || !samples_done) return fail(PST_EINVAL, "null plan or result");
+  *samples_done = p->range_pos;
+  return PST_OK;
+}
+
+int pst_plan_graph_create(pst_plan* p, int32_t* const* O_dev, const int64_t* o_strides, int32_t* const* flags_dev,
+                          const int32_t* f_ends, int32_t n_ranges) {
+  g_err[0] = 0;
+  if (!p) return fail(PST_EINVAL, "null plan");
+  if (!f_ends || n_ranges < 1) return fail(PST_EINVAL, "need at least one range end");
+  if (f_ends[n_ranges - 1] < max_ita(p))
+    return fail(PST_EINVAL, "the last range end %d must reach the last sample (%d)", f_ends[n_ranges - 1], max_ita(p));
+  for (int r = 0; r < n_ranges; ++r) {
+    const int rc = range_args_ok(p, f_ends[r], r > 0 ? f_ends[r - 1] : 0);
+    if (rc != PST_OK) return rc;
+  }
+  const bool ranges = n_ranges > 1;
+  if (ranges) {
+    const int rc = alloc_state(p);
+    if (rc != PST_OK) return rc;
+  }
+  hipError_t e;
+  // a capture cannot upload descriptors: prepare them now, on a stream of our own
+  hipStream_t cs = nullptr;
+  if ((e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  int count[pst::NUM_CLASSES];
+  int n_wave = 0;
+  int64_t wave_sims = 0;
+  int rc = prepare_plan(p, O_dev, o_strides, flags_dev, cs, ranges, count, &n_wave, &wave_sims);
+  if (rc == PST_OK && ((e = hipStreamSynchronize(cs)) != hipSuccess || (e = sync_plan(p, cs)) != hipSuccess))
+    rc = hip_fail(e, "hipStreamSynchronize");
+  if (rc != PST_OK) {
+    (void)hipStreamDestroy(cs);
+    return rc;
+  }
+  if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->graph_exec = nullptr;
+  p->graph = nullptr;
+  // every range's fill kernels and launches, in order, on the one stream:
+  // the graph holds kernel nodes only (the fills are kernels, not memsets)
+  if ((e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    return hip_fail(e, "hipStreamBeginCapture");
+  }
+  for (int r = 0; r < n_ranges && rc == PST_OK; ++r) {
+    const int n_jobs = (int)p->jobs.size();
+    if (r == 0) hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, cs, p->d_stats, (int)pst::ST_COUNT);
+    hipLaunchKernelGGL(zero_u32, dim3((n_jobs + 255) / 256), dim3(256), 0, cs, p->d_queue, n_jobs);
+    if ((e = hipGetLastError()) != hipSuccess) {
+      rc = hip_fail(e, "zero kernels");
+      break;
+    }
+    const pst::Range range = {ranges ? f_ends[r] : pst::FULL_RUN.f_end, r > 0 ? 1 : 0};
+    auto one_stream = [&]() -> hipStream_t { return cs; };
+    rc = enqueue_launches(p, count, n_wave, wave_sims, range, one_stream);
+  }
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(cs, &g);
+  if (rc == PST_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+  if (rc == PST_OK && (e = hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0)) != hipSuccess) {
+    p->graph_exec = nullptr;
+    rc = hip_fail(e, "hipGraphInstantiate");
+  }
+  (void)hipStreamDestroy(cs);
+  if (rc != PST_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  p->graph = g;
+  p->graph_generation = p->generation;
+  p->range_pos = 0;
+  return PST_OK;
+}
+
+int pst_plan_graph_launch(pst_plan* p, void* stream) {
+  g_err[0] = 0;
+  if (!p) return fail(PST_EINVAL, "null plan");
+  if (!p->graph_exec) return fail(PST_EINVAL, "no graph: call pst_plan_graph_create first");
+  if (p->graph_generation != p->generation)
+    return fail(PST_EINVAL, "the plan's descriptors or launch shape changed since the graph was captured: "
+                            "capture it again");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  p->executed = false;
+  if (p->enqueued && (e = hipStreamWaitEvent(s, p->ev1, 0)) != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
+  if ((e = hipEventRecord(p->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+  if ((e = hipGraphLaunch(p->graph_exec, s)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+  if ((e = hipEventRecord(p->ev1, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+  p->enqueued = true;
+  p->executed = true;
+  p->range_pos = 0;
   return PST_OK;
 }
 
@@ -1042,6 +1247,7 @@ int pst_plan_set_launch(pst_plan* p, int32_t lanes_shift, int32_t ssa_slice, int
   p->ssa_slice = ssa_slice;
   p->waves_trim = waves_trim;
   p->burst_waves = burst_waves;
+  p->generation += 1;
   return PST_OK;
 }
 
@@ -1052,6 +1258,7 @@ int pst_plan_set_stream(pst_plan* p, uint64_t seed, uint64_t sim_offset) {
     j.p.seed = seed;
     j.p.sim_offset = sim_offset;
   }
+  p->generation += 1;
   return PST_OK;
 }
 
@@ -1207,8 +1414,8 @@ int pst_run_noalloc(const pst_network* net, const pst_config* cfg, const double*
   const void* jobs = base + kNoAllocJob;
   unsigned int* queue = (unsigned int*)(base + kNoAllocQueue);
   unsigned long long* counters = (unsigned long long*)base;
-  rc = mixed ? pst::launch_mixed(cls, jobs, 1, queue, counters, waves, s)
-             : pst::launch_fp64(cls, jobs, 1, queue, counters, waves, s);
+  rc = mixed ? pst::launch_mixed(cls, jobs, 1, queue, counters, waves, pst::FULL_RUN, s)
+             : pst::launch_fp64(cls, jobs, 1, queue, counters, waves, pst::FULL_RUN, s);
   if (rc != 0) return hip_fail((hipError_t)rc, "simulation kernel launch");
   if (stats) {
     unsigned long long hs[pst::ST_COUNT];

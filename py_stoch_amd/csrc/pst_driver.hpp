@@ -137,6 +137,67 @@ struct SampleOut {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Sample ranges (pst_plan_execute_range, graph-captured chunks).  A launch
+// with f_end < ita stops each simulation once it has recorded sample
+// f_end - 1 and saves its state to JobParams::state, word-major
+// ([word][n_sims]): the engine registers, the mode, the simulation's events
+// so far and, for the blocked layout, its gathered samples of the block in
+// progress (the step that records sample f_end - 1 may record later samples
+// of the same block too); a launch with resume = 1 takes every simulation up
+// from there.  A simulation that ends saves MODE_DONE, so later ranges skip it.
+// Stopping between two steps and going on from the saved registers changes
+// nothing in a simulation's step sequence, so a run split into ranges gives
+// the same bits as one launch.  The trajectory base pointer is recomputed on
+// resume (a range may write into another buffer).  f_end = INT_MAX,
+// resume = 0: one launch runs every simulation to its end.
+// ---------------------------------------------------------------------------
+template <class Engine>
+struct StateIO {
+  using Out = typename Engine::Out;
+  static constexpr int EW = (int)(sizeof(Engine) / 4);
+  static constexpr int BW = Out::KS * OUT_BLOCK;  // the lane's LDS sample block words
+  static constexpr int WORDS = EW + 3 + BW;
+  static_assert(sizeof(Engine) % 4 == 0, "engine state is not whole words");
+  template <class P>
+  static __device__ __forceinline__ bool fits(const P& p) {
+    return p.state != nullptr && p.state_words >= WORDS;
+  }
+  template <class P>
+  static __device__ __forceinline__ void save(const P& p, int64_t sim, const Engine& eng, int mode,
+                                              unsigned long long ev) {
+    gptr<uint32_t> st = glb(p.state);
+    const int64_t n = p.n_sims;
+    uint32_t w[EW];
+    __builtin_memcpy(w, &eng, sizeof(Engine));
+#pragma unroll
+    for (int i = 0; i < EW; ++i) st[(int64_t)i * n + sim] = w[i];
+    st[(int64_t)EW * n + sim] = (uint32_t)mode;
+    st[(int64_t)(EW + 1) * n + sim] = (uint32_t)ev;
+    st[(int64_t)(EW + 2) * n + sim] = (uint32_t)(ev >> 32);
+    if (p.layout == LAYOUT_BLOCKS && mode != MODE_DONE) {
+#pragma unroll
+      for (int i = 0; i < BW; ++i) st[(int64_t)(EW + 3 + i) * n + sim] = (uint32_t)Out::lds()[i * 256 + Out::slot()];
+    }
+  }
+  template <class P>
+  static __device__ __forceinline__ int load(const P& p, int64_t sim, Engine& eng, unsigned long long& ev) {
+    gptr<uint32_t> st = glb(p.state);
+    const int64_t n = p.n_sims;
+    uint32_t w[EW];
+#pragma unroll
+    for (int i = 0; i < EW; ++i) w[i] = st[(int64_t)i * n + sim];
+    __builtin_memcpy(&eng, w, sizeof(Engine));
+    ev = ((unsigned long long)st[(int64_t)(EW + 2) * n + sim] << 32) | st[(int64_t)(EW + 1) * n + sim];
+    const int mode = (int)st[(int64_t)EW * n + sim];
+    if (p.layout == LAYOUT_BLOCKS && mode != MODE_DONE) {
+#pragma unroll
+      for (int i = 0; i < BW; ++i) Out::lds()[i * 256 + Out::slot()] = (int32_t)st[(int64_t)(EW + 3 + i) * n + sim];
+    }
+    return mode;
+  }
+};
+
 // Engine interface (see EngineF64 / EngineF32):
 //   int  start(job, sim)            -> initial mode, state loaded
 //   int  step<SSA_ONLY>(job, cnt, m) -> next mode (DONE may set flag)
@@ -144,8 +205,60 @@ struct SampleOut {
 constexpr int SSA_LANES_MIN = 1;
 
 template <class Engine, class Ref>
+__device__ __forceinline__ void finish_sim(const Ref& jr, const Engine& eng, int64_t sim, Counters& cnt,
+                                           unsigned long long ev_start) {
+  int32_t* fl = jr.p().flags;
+  if (fl) glb(fl)[sim] = eng.flag;
+  cnt.flagged += (eng.flag != 0);
+  cnt.fevents += (eng.flag != 0) ? cnt.events - ev_start : 0ull;
+}
+
+// After a step: record an ended simulation, or stop one that has recorded
+// the range's last sample.  Returns the lane's mode.
+template <class Engine, class Ref>
+__device__ __forceinline__ int settle(const Ref& jr, const Engine& eng, int64_t sim, Counters& cnt,
+                                      unsigned long long ev_start, int mode, int f_stop) {
+  using SIO = StateIO<Engine>;
+  if (mode == MODE_DONE) {
+    finish_sim(jr, eng, sim, cnt, ev_start);
+    if (SIO::fits(jr.p())) SIO::save(jr.p(), sim, eng, MODE_DONE, cnt.events - ev_start);
+    return MODE_DONE;
+  }
+  if (eng.F >= f_stop) {
+    if (SIO::fits(jr.p())) {
+      SIO::save(jr.p(), sim, eng, mode, cnt.events - ev_start);
+      return MODE_DONE;
+    }
+  }
+  return mode;
+}
+
+// A simulation taken from the queue: started, or resumed from its saved
+// state.  Returns its mode (MODE_DONE: nothing left to do in this range).
+template <class Engine, class Ref>
+__device__ __forceinline__ int begin_sim(const Ref& jr, Engine& eng, int64_t sim, Counters& cnt,
+                                         unsigned long long& ev_start, bool resume, int f_stop) {
+  using SIO = StateIO<Engine>;
+  if (resume && SIO::fits(jr.p())) {
+    unsigned long long ev;
+    const int mode = SIO::load(jr.p(), sim, eng, ev);
+    ev_start = cnt.events - ev;
+    eng.Ob = Engine::Out::base(jr.p(), sim);
+    return (mode == MODE_DONE || eng.F >= f_stop) ? MODE_DONE : mode;
+  }
+  ev_start = cnt.events;
+  return eng.start(jr, sim);
+}
+
+template <class Ref>
+__device__ __forceinline__ int range_stop(const Ref& jr, int f_end) {
+  return f_end < jr.p().ita ? f_end : jr.p().ita;
+}
+
+template <class Engine, class Ref>
 __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__ jobs, int n_jobs,
-                                           unsigned int* __restrict__ queue, unsigned long long* __restrict__ stats) {
+                                           unsigned int* __restrict__ queue, unsigned long long* __restrict__ stats,
+                                           int f_end, int resume) {
   const int lane = (int)(threadIdx.x & 63u);
   Counters cnt = {};
   Engine eng;
@@ -157,6 +270,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
 
   Ref jr;
   jr.load(jobs);
+  int f_stop = range_stop(jr, f_end);
 
   for (;;) {
     // ---- refill lanes whose simulation ended from the job's queue ----
@@ -175,8 +289,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
         const int64_t s = (int64_t)base + rank;
         if (s < n_sims) {
           sim = s;
-          ev_start = cnt.events;
-          mode = eng.start(jr, s);
+          mode = begin_sim(jr, eng, s, cnt, ev_start, resume != 0, f_stop);
         }
       }
     }
@@ -185,6 +298,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
       job += 1;
       if (job >= n_jobs) break;
       jr.load(jobs + job);
+      f_stop = range_stop(jr, f_end);
       exhausted = false;
       continue;
     }
@@ -193,12 +307,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
     if (wave_any(mode == MODE_DECIDE)) {
       if (mode != MODE_DONE) {
         mode = eng.template step<false>(jr, cnt, mode);
-        if (mode == MODE_DONE) {
-          int32_t* fl = jr.p().flags;
-          if (fl) glb(fl)[sim] = eng.flag;
-          cnt.flagged += (eng.flag != 0);
-          cnt.fevents += (eng.flag != 0) ? cnt.events - ev_start : 0ull;
-        }
+        mode = settle(jr, eng, sim, cnt, ev_start, mode, f_stop);
       }
       // SSA-only steps while lanes are inside a burst
       const int slice = jr.p().ssa_slice;
@@ -207,12 +316,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
         jr.refresh();
         if (mode == MODE_SSA) {
           mode = eng.template step<true>(jr, cnt, mode);
-          if (mode == MODE_DONE) {
-            int32_t* fl = jr.p().flags;
-            if (fl) glb(fl)[sim] = eng.flag;
-            cnt.flagged += (eng.flag != 0);
-            cnt.fevents += (eng.flag != 0) ? cnt.events - ev_start : 0ull;
-          }
+          mode = settle(jr, eng, sim, cnt, ev_start, mode, f_stop);
         }
       }
     } else {
@@ -225,12 +329,7 @@ __device__ __forceinline__ void run_driver(const typename Ref::Dev* __restrict__
         if (s > 0) jr.refresh();
         if (mode == MODE_SSA) {
           mode = eng.template step<true>(jr, cnt, mode);
-          if (mode == MODE_DONE) {
-            int32_t* fl = jr.p().flags;
-            if (fl) glb(fl)[sim] = eng.flag;
-            cnt.flagged += (eng.flag != 0);
-            cnt.fevents += (eng.flag != 0) ? cnt.events - ev_start : 0ull;
-          }
+          mode = settle(jr, eng, sim, cnt, ev_start, mode, f_stop);
         }
       }
     }
@@ -318,18 +417,9 @@ struct SlotIO {
 };
 
 template <class Engine, class Ref>
-__device__ __forceinline__ void finish_sim(const Ref& jr, const Engine& eng, int64_t sim, Counters& cnt,
-                                           unsigned long long ev_start) {
-  int32_t* fl = jr.p().flags;
-  if (fl) glb(fl)[sim] = eng.flag;
-  cnt.flagged += (eng.flag != 0);
-  cnt.fevents += (eng.flag != 0) ? cnt.events - ev_start : 0ull;
-}
-
-template <class Engine, class Ref>
 __device__ __forceinline__ void run_driver_paired(const typename Ref::Dev* __restrict__ jobs, int n_jobs,
                                                   unsigned int* __restrict__ queue,
-                                                  unsigned long long* __restrict__ stats) {
+                                                  unsigned long long* __restrict__ stats, int f_end, int resume) {
   using IO = SlotIO<Engine>;
   __shared__ uint32_t s_data[IO::WORDS * PAIR_SLOTS];
   __shared__ uint32_t s_state[PAIR_SLOTS];
@@ -355,13 +445,15 @@ __device__ __forceinline__ void run_driver_paired(const typename Ref::Dev* __res
     int job = 0;
     bool exhausted = false;
     jr.load(jobs);
+    int f_stop = range_stop(jr, f_end);
     for (;;) {
-      // a burst handed back
+      // a burst handed back (ended simulations were recorded by the burst lane)
       if (mode == MODE_AWAY && lds_load_acquire(&s_state[slot]) == SLOT_BACK) {
         unsigned long long ev;
         IO::get(s_data, slot, eng, sim, ev, mode);
         ev_start = cnt.events - ev;
         s_state[slot] = SLOT_FREE;
+        if (mode != MODE_DONE) mode = settle(jr, eng, sim, cnt, ev_start, mode, f_stop);
       }
       const bool want = (mode == MODE_DONE) && !exhausted && lane < jr.p().lanes;
       const unsigned long long wmask = __builtin_amdgcn_ballot_w64(want);
@@ -378,8 +470,7 @@ __device__ __forceinline__ void run_driver_paired(const typename Ref::Dev* __res
           const int64_t s = (int64_t)base + rank;
           if (s < n_sims) {
             sim = s;
-            ev_start = cnt.events;
-            mode = eng.start(jr, s);
+            mode = begin_sim(jr, eng, s, cnt, ev_start, resume != 0, f_stop);
           }
         }
       }
@@ -387,6 +478,7 @@ __device__ __forceinline__ void run_driver_paired(const typename Ref::Dev* __res
         job += 1;
         if (job >= n_jobs) break;
         jr.load(jobs + job);
+        f_stop = range_stop(jr, f_end);
         exhausted = false;
         continue;
       }
@@ -394,7 +486,7 @@ __device__ __forceinline__ void run_driver_paired(const typename Ref::Dev* __res
         jr.refresh();
         if (mode == MODE_DECIDE) {
           mode = eng.template step<false>(jr, cnt, mode);
-          if (mode == MODE_DONE) finish_sim(jr, eng, sim, cnt, ev_start);
+          mode = settle(jr, eng, sim, cnt, ev_start, mode, f_stop);
         }
       } else if (!wave_any(mode == MODE_SSA)) {
         __builtin_amdgcn_s_sleep(2);  // every live lane is away: wait for a burst to come back
@@ -413,6 +505,7 @@ __device__ __forceinline__ void run_driver_paired(const typename Ref::Dev* __res
     // wavefront's slowed the Lotka-Volterra workload from 39.9 to 42.3 ms.)
     bool busy = false;
     int bjob = -1;
+    int f_stop = 0x7fffffff;
     for (;;) {
       const bool pick = !busy && lds_load_acquire(&s_state[slot]) == SLOT_SENT;
       const unsigned long long pmask = __builtin_amdgcn_ballot_w64(pick);
@@ -422,6 +515,7 @@ __device__ __forceinline__ void run_driver_paired(const typename Ref::Dev* __res
         if (pj != bjob) {
           jr.load(jobs + pj);
           bjob = pj;
+          f_stop = range_stop(jr, f_end);
         }
         if (pick) {
           unsigned long long ev;
@@ -438,9 +532,11 @@ __device__ __forceinline__ void run_driver_paired(const typename Ref::Dev* __res
       }
       jr.refresh();
       if (busy) {
-        const int m = eng.template step<true>(jr, cnt, MODE_SSA);
-        if (m != MODE_SSA) {
-          if (m == MODE_DONE) finish_sim(jr, eng, sim, cnt, ev_start);
+        int m = eng.template step<true>(jr, cnt, MODE_SSA);
+        if (m == MODE_DONE) m = settle(jr, eng, sim, cnt, ev_start, m, f_stop);
+        // back to the decide lane at the burst's end, or at the range's
+        // last sample (the decide lane saves it)
+        if (m != MODE_SSA || eng.F >= f_stop) {
           IO::put(s_data, slot, eng, sim, cnt.events - ev_start, bjob, m);
           lds_store_release(&s_state[slot], SLOT_BACK);
           busy = false;

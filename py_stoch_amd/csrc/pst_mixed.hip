@@ -6,18 +6,18 @@ namespace pst {
 template <int NP, int MP>
 __global__ void __launch_bounds__(256) sim_mixed_kernel(const JobDev<NP, MP>* __restrict__ jobs, int n_jobs,
                                                         unsigned int* __restrict__ queue,
-                                                        unsigned long long* __restrict__ stats) {
-  run_driver<EngineF32<JobRef<NP, MP>>, JobRef<NP, MP>>(jobs, n_jobs, queue, stats);
+                                                        unsigned long long* __restrict__ stats, Range range) {
+  run_driver<EngineF32<JobRef<NP, MP>>, JobRef<NP, MP>>(jobs, n_jobs, queue, stats, range.f_end, range.resume);
 }
 
 template <int NP, int MP>
-static int launch_cls_m(const void* jobs, int n_jobs, unsigned int* queue, unsigned long long* stats, int waves,
+static int launch_cls_m(const void* jobs, int n_jobs, unsigned int* queue, unsigned long long* stats, int waves, Range range,
                         hipStream_t stream) {
   const int block = 256;
   const int grid = (waves + 3) / 4;
   if (grid <= 0) return 0;
   hipLaunchKernelGGL((sim_mixed_kernel<NP, MP>), dim3(grid), dim3(block), 0, stream,
-                     (const JobDev<NP, MP>*)jobs, n_jobs, queue, stats);
+                     (const JobDev<NP, MP>*)jobs, n_jobs, queue, stats, range);
   return (int)hipGetLastError();
 }
 
@@ -29,11 +29,11 @@ static int occupancy_cls_m() {
 }
 
 int launch_mixed(int cls, const void* jobs, int n_jobs, unsigned int* queue, unsigned long long* stats, int waves,
-                 hipStream_t stream) {
+                 Range range, hipStream_t stream) {
   switch (cls) {
-    case 0: return launch_cls_m<4, 4>(jobs, n_jobs, queue, stats, waves, stream);
-    case 1: return launch_cls_m<8, 8>(jobs, n_jobs, queue, stats, waves, stream);
-    case 2: return launch_cls_m<16, 16>(jobs, n_jobs, queue, stats, waves, stream);
+    case 0: return launch_cls_m<4, 4>(jobs, n_jobs, queue, stats, waves, range, stream);
+    case 1: return launch_cls_m<8, 8>(jobs, n_jobs, queue, stats, waves, range, stream);
+    case 2: return launch_cls_m<16, 16>(jobs, n_jobs, queue, stats, waves, range, stream);
     default: return -1;
   }
 }

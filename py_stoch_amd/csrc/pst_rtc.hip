@@ -177,9 +177,10 @@ std::string rtc_source(const HostNet& net, const JobParams& c, int precision,
   if (waves_per_eu > 0) s += " __attribute__((amdgpu_waves_per_eu(" + std::to_string(waves_per_eu) + ")))";
   s += "\n"
        "PST_KERNEL_NAME(const pst::JobParams* __restrict__ jobs, int n_jobs, unsigned int* __restrict__ queue,\n"
-       "                 unsigned long long* __restrict__ stats) {\n";
+       "                 unsigned long long* __restrict__ stats, pst::Range range) {\n";
   s += std::string("  pst::") + (paired ? "run_driver_paired" : "run_driver") +
-       "<pst::PST_ENGINE<pst::FixedRef<pst::NetGen>>, pst::FixedRef<pst::NetGen>>(jobs, n_jobs, queue, stats);\n}\n";
+       "<pst::PST_ENGINE<pst::FixedRef<pst::NetGen>>, pst::FixedRef<pst::NetGen>>(jobs, n_jobs, queue, stats, "
+       "range.f_end, range.resume);\n}\n";
   return s;
 }
 
@@ -247,10 +248,10 @@ int rtc_kernel(const std::string& source, int precision, RtcKernel* out, std::st
 }
 
 hipError_t rtc_launch(const RtcKernel& k, const JobParams* jobs, int n_jobs, unsigned int* queue,
-                      unsigned long long* stats, int waves, hipStream_t stream) {
+                      unsigned long long* stats, int waves, Range range, hipStream_t stream) {
   const int grid = (waves + 3) / 4;  // four simulation wavefronts per workgroup
   if (grid <= 0) return hipSuccess;
-  void* args[] = {(void*)&jobs, (void*)&n_jobs, (void*)&queue, (void*)&stats};
+  void* args[] = {(void*)&jobs, (void*)&n_jobs, (void*)&queue, (void*)&stats, (void*)&range};
   return hipModuleLaunchKernel(k.fn, grid, 1, 1, k.block, 1, 1, 0, stream, args, nullptr);
 }
 

@@ -49,6 +49,6 @@ int rtc_kernel(const std::string& source, int precision, RtcKernel* out, std::st
 // Launch a specialised kernel: `jobs` is a device array of n_jobs JobParams;
 // `waves` simulation wavefronts, four per workgroup.
 hipError_t rtc_launch(const RtcKernel& k, const JobParams* jobs, int n_jobs, unsigned int* queue,
-                      unsigned long long* stats, int waves, hipStream_t stream);
+                      unsigned long long* stats, int waves, Range range, hipStream_t stream);
 
 }  // namespace pst

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_function(native):
 
 
 def test_abi_version_and_limits(native):
-    assert native.load().pst_abi_version() == 6 == native.ABI_VERSION
+    assert native.load().pst_abi_version() == 7 == native.ABI_VERSION
     assert native.max_network() == (64, 256)
 
 
@@ -107,6 +107,20 @@ def test_autotune_argument_checks(native):
 def test_set_launch_argument_checks(native):
     L = native.load()
     assert L.pst_plan_set_launch(None, 0, 7, 0, 0) == -1  # PST_EINVAL: null plan
+
+
+def test_range_and_graph_argument_checks(native):
+    """The sample-range and graph entry points reject a null plan before
+    touching a device."""
+    import ctypes
+    L = native.load()
+    pos = ctypes.c_int32()
+    assert L.pst_plan_execute_range(None, None, None, None, 8, None) == -1
+    assert L.pst_plan_range_position(None, ctypes.byref(pos)) == -1
+    assert L.pst_plan_graph_create(None, None, None, None, None, 0) == -1
+    assert L.pst_plan_graph_launch(None, None) == -1
+    assert L.pst_blocks_to_rows(None, 0, None, 0, 1, 8, 4, None) == -1  # strides below n_sims
+    assert b"strides" in L.pst_last_error()
 
 
 def test_noalloc_argument_checks(native):
