@@ -163,3 +163,81 @@ def test_range_argument_checks(native):
         pb.close()
     finally:
         O.free()
+
+
+def _batch_jobs(native, layout):
+    """Three FP64 jobs (one precision per batch) on three kernels:
+    Lotka-Volterra on the table kernel, Schlogl (fewer samples) on its network
+    kernel, and a Lotka-Volterra sweep of two parameter rows with a step
+    budget on the Lotka-Volterra network kernel."""
+    lv = models.make_args(models.lotka_volterra(), E=[0, 1], t_max=5.0, ita=96, U=3000)
+    sc = models.make_args(models.schlogl(), E=[0], t_max=5.0, ita=40, U=2000)
+    lv_t, sc_t = tables_of(lv), tables_of(sc)
+    rates2 = np.stack([lv.rate_fn(lv.c), lv.rate_fn(lv.c) * np.array([1.2, 0.9, 1.1])])
+    x02 = np.stack([lv.x_0, lv.x_0 // 2])
+    return [
+        native.Job(lv_t, lv.rate_fn(lv.c), lv.x_0, method=0, t_max=5.0, I=lv.I, E=lv.E, n_sims=3000, seed=4,
+                   precision=0, layout=layout),
+        native.Job(sc_t, sc.rate_fn(sc.c), sc.x_0, method=0, t_max=5.0, I=sc.I, E=sc.E, n_sims=2000, seed=5,
+                   precision=0, specialize=True, layout=layout),
+        native.Job(lv_t, rates2, x02, method=0, t_max=5.0, I=lv.I, E=lv.E, n_sims=4000, sims_per_set=2000,
+                   seed=6, precision=0, specialize=True, layout=layout, max_steps=4000),
+    ]
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_batch_plan_ranges_bit_exact(native, layout):
+    """A batch plan split into ranges (jobs of different sample counts: a
+    range past a job's last sample ends that job) equals one execute of the
+    same batch: trajectories, flags and summed counters."""
+    ends = [16, 40, 64, 96]
+    out = []
+    for split in (False, True):
+        jobs = _batch_jobs(native, layout)
+        shapes = [_shape(layout, j.kappa, j.ita, j.n_sims) for j in jobs]
+        bufs = [DeviceArray(s) for s in shapes]
+        flags = [DeviceArray((j.n_sims,)) for j in jobs]
+        try:
+            p = native.BatchPlan(jobs)
+            O_ptrs, strides, F_ptrs = [b.ptr.value for b in bufs], [j.n_sims for j in jobs], [f.ptr.value for f in flags]
+            print("case: batch ranges, layout %d, split %s" % (layout, split), flush=True)
+            if not split:
+                p.execute(O_ptrs, strides, F_ptrs)
+                st = p.stats()
+            else:
+                st = {k: 0 for k in _KEYS}
+                for f in ends:
+                    p.execute_range(O_ptrs, strides, f, F_ptrs)
+                    s = p.stats()
+                    for k in _KEYS:
+                        st[k] += s[k]
+                assert p.range_position() == 0
+            out.append(([b.get() for b in bufs], [f.get() for f in flags], st))
+            p.close()
+        finally:
+            for b in bufs + flags:
+                b.free()
+    (O1, F1, s1), (O2, F2, s2) = out
+    for q in range(3):
+        assert np.array_equal(O1[q], O2[q]), "job %d trajectories differ" % q
+        assert np.array_equal(F1[q], F2[q]), "job %d flags differ" % q
+    assert all(s1[k] == s2[k] for k in _KEYS), [(k, s1[k], s2[k]) for k in _KEYS]
+    assert s2["ssa_steps"] > 0 and s2["leaps"] > 0
+
+
+def test_batch_ranges_refuse_large_network_job(native):
+    """Sample ranges run the lane kernels only: a batch with a job for the
+    large-network kernel refuses them before any launch."""
+    rng = np.random.default_rng(3)
+    big = models.make_args(models.random_network(rng, 24, 70), E=[0, 1], t_max=0.2, ita=16, U=256)
+    job = native.Job(tables_of(big), big.rate_fn(big.c), big.x_0, method=0, t_max=0.2, I=big.I, E=big.E,
+                     n_sims=256, seed=2, precision=1, max_steps=2000)
+    O, F = DeviceArray((2, 16, 256)), DeviceArray((256,))
+    try:
+        p = native.BatchPlan([job])
+        with pytest.raises(native.NativeError):
+            p.execute_range([O.ptr.value], [256], 8, [F.ptr.value])
+        p.close()
+    finally:
+        O.free()
+        F.free()
