@@ -7,6 +7,11 @@ The FP64 batch is bit-exact against the oracle on the first 8 simulations of
 every set (same global simulation ids, so the same random streams), and
 mixed precision agrees with FP64 set by set in distribution.
 
+C3: workloads.schlogl() in full -- 262,144 simulations, 100 samples, one
+job on the network kernels.  FP64 is bit-exact against the oracle on the
+first and the last 16,384 simulations (same global simulation ids), and
+mixed precision agrees with FP64 at every sample time in distribution.
+
 C5: eight networks of workloads.random_networks() -- the largest one (49
 species, 193 reactions) and networks 0-6 -- under the same step budget
 (20,000 leap decisions + SSA steps per simulation), mixed precision against
@@ -125,3 +130,36 @@ def test_random_networks_mixed_vs_oracle(native, oracle_mod):
                 assert d < crit, (i, q, f, d, crit)
     print("random networks: %d species/sample comparisons, worst KS / critical %.2f" % (checks, worst), flush=True)
     assert checks > 300
+
+
+def test_schlogl_full_fp64_bit_exact_and_mixed(native, oracle_mod):
+    spec = workloads.schlogl()[0]
+    U = spec.n_sims
+    assert (U, spec.ita) == (262144, 100)
+    print("case: schlogl C3 fp64, %d simulations" % U, flush=True)
+    (O64,), (F64,), st64 = native.batch_simulate(workloads.jobs([spec], native.PRECISION_FP64))
+    print("case: schlogl C3 mixed", flush=True)
+    (Omx,), (Fmx,), stmx = native.batch_simulate(workloads.jobs([spec], native.PRECISION_MIXED))
+    assert st64["ssa_steps"] > 0 and st64["leaps"] > 0 and stmx["ssa_steps"] > 0
+    assert not F64.any() and not Fmx.any()
+    part = 16384
+    for off in (0, U - part):
+        print("case: schlogl C3 oracle on simulations %d..%d" % (off, off + part - 1), flush=True)
+        kw = dict(spec.kw)
+        kw.update(n_sims=part, sims_per_set=part, sim_offset=off)
+        Oc, Fc, _ = oracle_mod.simulate(spec.tables, spec.rates, spec.x0, threads=THREADS, **kw)
+        sl = slice(off, off + part)
+        bad = np.argwhere(O64[:, :, sl] != Oc)
+        assert bad.shape[0] == 0, "FP64 GPU differs from the oracle at %s" % bad[:3].tolist()
+        assert np.array_equal(F64[sl], Fc)
+    crit = _ks_crit(U, U)
+    worst = 0.0
+    for f in range(1, spec.ita):
+        d = laws.ks2(Omx[0, f].astype(np.float64), O64[0, f].astype(np.float64))
+        worst = max(worst, d)
+        assert d < crit, (f, d, crit)
+    x, y = Omx[0, -1].astype(np.float64), O64[0, -1].astype(np.float64)
+    z = abs(x.mean() - y.mean()) / np.sqrt(x.var() / U + y.var() / U)
+    print("schlogl C3: max KS mixed vs FP64 %.4f (critical %.4f), |z| of the final mean %.2f" % (worst, crit, z),
+          flush=True)
+    assert z < 5
