@@ -270,7 +270,11 @@ int pst_plan_set_launch(pst_plan* plan, int32_t lanes_shift, int32_t ssa_slice, 
  * previous range stopped (from the start if no range is pending) until it
  * has recorded samples 0 .. f_end - 1 of O_dev (or ended); once f_end
  * reaches every job's last sample the run is complete and the next range
- * starts afresh.  Jobs writing PST_LAYOUT_BLOCKS need f_end to be a multiple
+ * starts afresh.  O_dev holds the whole trajectory (every sample index) and
+ * should be the same array for every range of a run: the step that records
+ * sample f_end - 1 may record later samples too, which the rows layout
+ * writes at once (into that range's O_dev) and the blocked layout keeps
+ * with the simulation until its block is complete.  Jobs writing PST_LAYOUT_BLOCKS need f_end to be a multiple
  * of 8 (or the last sample).  Large-network jobs are not supported.
  * pst_plan_execute and pst_plan_autotune abandon a pending range sequence.
  * pst_plan_range_position reports the samples recorded so far (0: none

@@ -149,7 +149,7 @@ struct SampleOut {
 // Stopping between two steps and going on from the saved registers changes
 // nothing in a simulation's step sequence, so a run split into ranges gives
 // the same bits as one launch.  The trajectory base pointer is recomputed on
-// resume (a range may write into another buffer).  f_end = INT_MAX,
+// resume from the launch's own descriptor.  f_end = INT_MAX,
 // resume = 0: one launch runs every simulation to its end.
 // ---------------------------------------------------------------------------
 // A value the optimizer may not reason about: the word offsets i * n of the
