@@ -443,9 +443,12 @@ class Plan(_RangeMixin):
                                           int(burst_waves)),
                "pst_plan_set_launch")
 
-    def autotune(self, O_dev_ptr, o_stride, flags_dev_ptr=None, stream_ptr=None, fraction=0.25):
+    def autotune(self, O_dev_ptr, o_stride, flags_dev_ptr=None, stream_ptr=None, fraction=1.0):
         """Time the candidate launch shapes on the first `fraction` of the
-        sample times (writing into O_dev_ptr) and keep the fastest; returns it."""
+        sample times (writing into O_dev_ptr) and keep the fastest; returns it.
+        The whole horizon by default: the work per sample interval can grow
+        over a run (Lotka-Volterra's later samples cost ~3x the first), and a
+        shape timed on an early part only can be slower on the whole."""
         return _autotune(self._h, [O_dev_ptr], [o_stride], [flags_dev_ptr] if flags_dev_ptr else None,
                          stream_ptr, fraction)
 
@@ -531,9 +534,10 @@ class BatchPlan(_RangeMixin):
                                           int(burst_waves)),
                "pst_plan_set_launch")
 
-    def autotune(self, O_ptrs, strides, flag_ptrs=None, stream_ptr=None, fraction=0.25):
+    def autotune(self, O_ptrs, strides, flag_ptrs=None, stream_ptr=None, fraction=1.0):
         """Time the candidate launch shapes on the first `fraction` of every
-        job's sample times and keep the fastest; returns it."""
+        job's sample times (the whole horizon by default) and keep the
+        fastest; returns it."""
         return _autotune(self._h, O_ptrs, strides, flag_ptrs, stream_ptr, fraction)
 
     def set_stream(self, seed, sim_offset=0):
