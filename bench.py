@@ -54,6 +54,9 @@ COPY_BYTES = 256 << 20
 # half only (15 SSA steps per pass: 4 % faster there) was 7 % slower overall
 TUNE_FRACTION = 1.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_lv65536_mixed.txt")
+ISSUE_FILE = os.path.join(ROOT, "profiles", "pmc_lv65536_mixed_instruction_mix.txt")
+SIMDS = 256 * 4
+CLOCK_HZ = 2.4e9
 
 
 def read_traffic(path=TRAFFIC_FILE):
@@ -68,6 +71,27 @@ def read_traffic(path=TRAFFIC_FILE):
     except OSError:
         pass
     return None
+
+
+def read_issue(kernel_ms, path=ISSUE_FILE):
+    """Issue-slot use of the mixed kernel: (VALU + SALU + branch instructions
+    per SIMD, from the committed SQ counter summary of one launch) over the
+    SIMD's quad-cycles in this run's kernel time at 2.4 GHz.  A wavefront
+    issues at most one instruction per quad-cycle, so this is the fraction
+    of the issue bound the kernel reaches."""
+    try:
+        cnt = {}
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 2 and parts[0].startswith("SQ_"):
+                    cnt[parts[0]] = float(parts[1])
+        ins = (cnt["SQ_INSTS_VALU"] + cnt["SQ_INSTS_SALU"] + cnt["SQ_INSTS_BRANCH"]) / SIMDS
+    except (OSError, KeyError):
+        return None
+    quad = kernel_ms * 1e-3 * CLOCK_HZ / 4.0
+    return {"frac": ins / quad, "instructions_per_simd": ins, "quad_cycles": quad,
+            "counters": os.path.relpath(path, ROOT)}
 
 
 def work_model(tabs, kappa, st):
@@ -598,6 +622,7 @@ def main():
             "hbm": {"algorithmic_bytes": traj_bytes, "achieved_GBps": traj_bytes / t_k / 1e9,
                     "frac": traj_bytes / t_k / HBM_PEAK}}
     roof["traffic"] = read_traffic()
+    roof["issue"] = read_issue(kern_ms_m)
 
     cpu = None
     if not args.no_cpu:
