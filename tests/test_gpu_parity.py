@@ -139,3 +139,31 @@ def test_step_budget_bit_exact(native, oracle_mod, specialize):
         assert np.array_equal(O_g, O_c)
         for k in ("events", "leaps", "ssa_steps", "flagged", "flagged_events"):
             assert s_g[k] == s_c[k], k
+
+
+@pytest.mark.parametrize("specialize", [False, True])
+def test_infinite_tau_is_flagged(native, oracle_mod, specialize):
+    """Immigration of X beside a decay of Y that starts critical (Y0 = 5 <
+    n_c): once Y is extinct no species is a reactant of a non-critical
+    reaction (tau_1 = inf, P1_P2.py:390-394) and no critical propensity is
+    left (tau_2 = inf), where the reference would draw curand_poisson(inf).
+    The simulation stops with PST_FLAG_TAU_INF and zeros after its last
+    sample, as in the oracle: FP64 bit-exact, mixed in proportion."""
+    spn = models._spn(2, [({}, {0: 1}, "parameter_0"), ({1: 1}, {}, "parameter_1*species_1")], [0, 5], [10.0, 1.0])
+    U, ita = 2048, 51
+    a = models.make_args(spn, E=[0, 1], t_max=5.0, ita=ita, U=U)
+    kw = dict(method=0, t_max=5.0, I=a.I, E=a.E, n_sims=U, seed=7)
+    print("case: infinite tau, specialize=%s" % specialize, flush=True)
+    O_g, f_g, s_g = native.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, specialize=specialize, **kw)
+    O_c, f_c, s_c = oracle_mod.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, **kw)
+    assert set(np.unique(f_c)) == {0, native.FLAG_TAU_INF}
+    assert np.array_equal(f_g, f_c) and np.array_equal(O_g, O_c)
+    for k in ("events", "leaps", "flagged", "flagged_events"):
+        assert s_g[k] == s_c[k], k
+    flagged = f_c != 0
+    assert 0.5 < flagged.mean() < 1.0 and not O_c[:, -1, flagged].any()
+    O_m, f_m, _ = native.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, specialize=specialize, precision=1, **kw)
+    assert set(np.unique(f_m)) <= {0, native.FLAG_TAU_INF}
+    p, q = (f_m != 0).mean(), flagged.mean()
+    assert abs(p - q) < 5 * np.sqrt(q * (1 - q) / U) + 0.01, (p, q)
+    assert not O_m[:, -1, f_m != 0].any()

@@ -197,3 +197,32 @@ def test_flagged_events_are_the_events_of_flagged_simulations(oracle_mod):
         flagged += s1["events"] if f1[0] != 0 else 0
         assert s1["flagged_events"] == (s1["events"] if f1[0] != 0 else 0)
     assert total == st["events"] and flagged == st["flagged_events"] > 0
+
+
+def test_infinite_tau_flag_matches_extinction_law(oracle_mod):
+    """Immigration of X beside the decay of Y from Y0 = 5 < n_c (every decay
+    critical, so tau_1 = inf throughout): a simulation is flagged
+    FLAG_TAU_INF exactly when Y is extinct before t_end -- no reactant of a
+    non-critical reaction and no critical propensity left -- which for a pure
+    death process has probability (1 - e^(-t))^5; before the flag its
+    samples hold the state (Y only decreases), after it zeros.  (An
+    unflagged simulation can still show Y = 0 at t_end: the last sample is
+    interpolated inside the step that crosses it, P1_P2.py:542-584.)"""
+    spn = models._spn(2, [({}, {0: 1}, "parameter_0"), ({1: 1}, {}, "parameter_1*species_1")], [0, 5], [10.0, 1.0])
+    U, t_end = 20000, 2.0
+    a = models.make_args(spn, E=[0, 1], t_max=t_end, ita=21, U=U)
+    O, flags, st = oracle_mod.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, method=oracle_mod.METHOD_TAU_LEAPING,
+                                       t_max=t_end, I=a.I, E=a.E, n_sims=U, seed=5)
+    assert set(np.unique(flags)) <= {0, oracle_mod.FLAG_TAU_INF}
+    p = (1.0 - np.exp(-t_end)) ** 5
+    f = (flags != 0).mean()
+    assert abs(f - p) < 5 * np.sqrt(p * (1 - p) / U), (f, p)
+    Y = O[1]
+    ok = flags == 0
+    assert (np.diff(Y[:, ok], axis=0) <= 0).all() and (Y[0] == 5).all()
+    # a flagged simulation: its samples are the state up to the flag, zeros after
+    for s in np.nonzero(~ok)[0][:200]:
+        nz = np.nonzero(O[:, :, s].any(axis=0))[0]
+        last = nz.max() + 1 if nz.size else 0
+        assert not O[:, last:, s].any()
+        assert (np.diff(Y[:last, s]) <= 0).all()
