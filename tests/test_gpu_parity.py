@@ -167,3 +167,24 @@ def test_infinite_tau_is_flagged(native, oracle_mod, specialize):
     p, q = (f_m != 0).mean(), flagged.mean()
     assert abs(p - q) < 5 * np.sqrt(q * (1 - q) / U) + 0.01, (p, q)
     assert not O_m[:, -1, f_m != 0].any()
+
+
+@pytest.mark.parametrize("specialize", [False, True])
+def test_irregular_sample_times(native, oracle_mod, specialize):
+    """Sample times that are not a uniform grid: clusters closer than one leap
+    (a step crosses several at once, each interpolated on its own fraction,
+    P1_P2.py:542-584), repeated times (the same state twice) and long gaps;
+    FP64 bit-exact with the oracle."""
+    rng = np.random.default_rng(21)
+    I = np.sort(np.concatenate([[0.0], rng.uniform(0.0, 10.0, 40), np.full(3, 2.5), 5.0 + 1e-4 * np.arange(6)]))
+    I = I.astype(np.float32)
+    U = 1024
+    a = models.make_args(models.lotka_volterra(), E=[0, 1], t_max=10.0, ita=I.size, U=U)
+    kw = dict(method=0, t_max=10.0, I=I, E=a.E, n_sims=U, seed=4)
+    print("case: irregular sample times, specialize=%s" % specialize, flush=True)
+    O_g, f_g, s_g = native.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, specialize=specialize, **kw)
+    O_c, f_c, s_c = oracle_mod.simulate(tables_of(a), a.rate_fn(a.c), a.x_0, **kw)
+    assert np.array_equal(f_g, f_c) and np.array_equal(O_g, O_c)
+    assert s_g["events"] == s_c["events"] and s_g["leaps"] == s_c["leaps"]
+    same = np.nonzero(np.diff(I) == 0)[0]
+    assert same.size >= 2 and np.array_equal(O_c[:, same], O_c[:, same + 1])
