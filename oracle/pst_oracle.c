@@ -17,9 +17,15 @@
  *
  * Build: oracle/Makefile (gcc -O3 -ffp-contract=off -fopenmp, no fast-math).
  *
- * Pinning (tests/test_oracle_*.py): Philox known-answer vectors (Random123),
- * log/exp against numpy to <= 2 ulp, Poisson sampler against scipy pmf,
- * birth-death SSA against the exact Poisson law, tau selection known answers.
+ * Pinning (tests/test_oracle_*.py, tests/test_golden.py): Philox known-answer
+ * vectors (Random123); log/exp against numpy to <= 2 ulp; the Poisson sampler
+ * against the scipy pmf (a planted off-by-one fails); SSA against exact laws
+ * (birth-death Poisson law, first-order chain binomials, the Schlogl master
+ * equation); the leap decision against known answers and every CalculateG
+ * branch; the critical-reaction branch (tau_2, SingleCriticalReaction)
+ * against exponential and competing-decay laws (planted mistakes fail); tau
+ * halving; the step budget; the infinite-tau flag against an extinction law;
+ * committed golden trajectories of three seeded workloads.
  */
 #include <stdint.h>
 #include <string.h>
